@@ -1,0 +1,643 @@
+// Host side of the MI355X ALS engine: owns device buffers, builds the
+// per-side work schedule, launches the kernels and implements the C ABI of
+// include/als.h.
+#include "als.h"
+#include "als_launch.hpp"
+#include "als_sched.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+using namespace als;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Side {
+    int64_t rows = 0;           // rows of the matrix this side updates
+    int *ptr = nullptr, *idx = nullptr;
+    float *val = nullptr;
+    int *order = nullptr, *batch = nullptr;
+    int nbatches = 0;
+    int grid = 0;
+    float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
+};
+
+}  // namespace
+
+struct als_engine {
+    als_config cfg{};
+    int T = 0, ld = 0, prec = PREC_FP32, solve = SOLVE_CHOL, lds = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    std::string err;
+    int num_cu = 0;
+
+    int64_t m = 0, n = 0, nnz = 0, n_test = 0;
+    Side side[2];                      // 0: users (CSR), 1: items (CSC)
+    int *coo_row = nullptr;            // train COO rows (train RMSE)
+    int *t_row = nullptr, *t_col = nullptr;
+    float *t_val = nullptr;
+    float *U = nullptr, *V = nullptr;  // FP32 factors, row stride ld
+    uint16_t *Us = nullptr, *Vs = nullptr;  // 16-bit shadows (mixed precision)
+    int *queue = nullptr;              // 2 ints
+    int *fail = nullptr;               // 1 int
+    double *partials = nullptr, *dsum = nullptr;
+    int rmse_blocks = 1024;
+    float *Lsave = nullptr;
+    size_t Lsave_bytes = 0;
+    uint16_t *tmp16 = nullptr;         // shadow of caller-provided Y (forward_dev)
+    size_t tmp16_bytes = 0;
+    hipGraphExec_t graph = nullptr;
+    std::vector<void *> allocs;
+    bool have_train = false, have_test = false;
+};
+
+namespace {
+
+int set_err(als_engine *e, int code, const std::string &msg) {
+    if (e) e->err = msg;
+    return code;
+}
+
+#define HIPCHK(e, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t _st = (expr);                                                               \
+        if (_st != hipSuccess)                                                                 \
+            return set_err((e), ALS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_st)); \
+    } while (0)
+
+template <class T>
+int dev_alloc(als_engine *e, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void *q = nullptr;
+    HIPCHK(e, hipMalloc(&q, count * sizeof(T)));
+    e->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return ALS_OK;
+}
+
+void dev_free(als_engine *e, void *p) {
+    if (!p) return;
+    auto it = std::find(e->allocs.begin(), e->allocs.end(), p);
+    if (it != e->allocs.end()) e->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+int blocks_for_T(int f) { return (f + 1 + 15) / 16; }
+
+int check_csr(als_engine *e, int64_t rows, int64_t cols, int64_t nnz, const int *ptr, const int *idx,
+              const char *what) {
+    if (ptr[0] != 0 || (int64_t)ptr[rows] != nnz)
+        return set_err(e, ALS_E_INVALID, std::string(what) + ": pointers must start at 0 and end at nnz");
+    for (int64_t r = 0; r < rows; r++)
+        if (ptr[r + 1] < ptr[r]) return set_err(e, ALS_E_INVALID, std::string(what) + ": pointers decrease");
+    for (int64_t k = 0; k < nnz; k++)
+        if (idx[k] < 0 || idx[k] >= cols) return set_err(e, ALS_E_INVALID, std::string(what) + ": index out of range");
+    return ALS_OK;
+}
+
+int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr) {
+    Schedule sc = build_schedule(rows, ptr, e->cfg.batch_work, s.grid, e->prec);
+    HIPCHK(e, hipMemcpyAsync(s.order, sc.order.data(), sc.order.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    int rc = dev_alloc(e, &s.batch, sc.batch_begin.size());
+    if (rc) return rc;
+    HIPCHK(e, hipMemcpyAsync(s.batch, sc.batch_begin.data(), sc.batch_begin.size() * sizeof(int),
+                             hipMemcpyHostToDevice, e->stream));
+    s.nbatches = (int)sc.batch_begin.size() - 1;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ALS_OK;
+}
+
+int drop_graph(als_engine *e) {
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    return ALS_OK;
+}
+
+// Launch one half-sweep. X/Y/shadow pointers and lambda are explicit so the
+// same path serves the engine's own factors and caller buffers.
+int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
+                const float *lambda_rows, float *Lsave) {
+    Side &s = e->side[sd];
+    UpdateArgs a{};
+    a.ptr = s.ptr;
+    a.idx = s.idx;
+    a.val = s.val;
+    a.Y = Y;
+    a.Ys = Ys;
+    a.X = X;
+    a.Xs = Xs;
+    a.lambda_rows = lambda_rows;
+    a.lambda = e->cfg.lambda;
+    a.f = e->cfg.f;
+    a.order = s.order;
+    a.batch_begin = s.batch;
+    a.nbatches = s.nbatches;
+    a.queue = e->queue;
+    a.cg_iters = e->cfg.cg_iters;
+    a.cg_tol = e->cfg.cg_tol;
+    a.cg_fp16_a = e->cfg.solve_type == ALS_SOLVE_CG_FP16;
+    a.refine = e->solve == SOLVE_CHOL ? e->cfg.refine_steps : 0;
+    a.fail_count = e->fail;
+    a.Lsave = Lsave;
+    if (s.rows == 0 || s.nbatches == 0) return ALS_OK;
+    HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, e->stream));
+    return ALS_OK;
+}
+
+int update_side(als_engine *e, int sd) {
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "als_set_train has not been called");
+    if (sd == ALS_SIDE_USERS)
+        return launch_side(e, 0, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr);
+    return launch_side(e, 1, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr);
+}
+
+int sweep_direct(als_engine *e) {
+    int rc = update_side(e, ALS_SIDE_USERS);
+    if (rc) return rc;
+    return update_side(e, ALS_SIDE_ITEMS);
+}
+
+int rmse_launch(als_engine *e, int which, double *host_out) {
+    RmseArgs a{};
+    if (which == ALS_RMSE_TEST) {
+        if (!e->have_test) return set_err(e, ALS_E_STATE, "als_set_test has not been called");
+        a.row = e->t_row;
+        a.col = e->t_col;
+        a.val = e->t_val;
+        a.n = e->n_test;
+    } else {
+        if (!e->have_train) return set_err(e, ALS_E_STATE, "als_set_train has not been called");
+        a.row = e->coo_row;
+        a.col = e->side[0].idx;
+        a.val = e->side[0].val;
+        a.n = e->nnz;
+    }
+    if (a.n == 0) {
+        *host_out = std::nan("");
+        return ALS_OK;
+    }
+    a.U = e->U;
+    a.V = e->V;
+    a.f = e->cfg.f;
+    a.ld = e->ld;
+    a.partials = e->partials;
+    HIPCHK(e, launch_rmse(a, e->rmse_blocks, e->dsum, e->stream));
+    double s = 0.0;
+    HIPCHK(e, hipMemcpyAsync(&s, e->dsum, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    *host_out = std::sqrt(s / (double)a.n);
+    return ALS_OK;
+}
+
+int refresh_shadow(als_engine *e, int sd) {
+    if (e->prec == PREC_FP32) return ALS_OK;
+    const int64_t rows = sd == 0 ? e->m : e->n;
+    const float *src = sd == 0 ? e->U : e->V;
+    uint16_t *dst = sd == 0 ? e->Us : e->Vs;
+    HIPCHK(e, launch_to16(e->prec, src, dst, rows * e->ld, e->stream));
+    return ALS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int als_config_init(als_config *c) {
+    if (!c) return ALS_E_INVALID;
+    std::memset(c, 0, sizeof(*c));
+    c->f = 100;
+    c->lambda = 0.05f;
+    c->iters = 10;
+    c->solve_type = ALS_SOLVE_CHOL;
+    c->gram_prec = ALS_PREC_FP32;
+    c->cg_iters = 6;
+    c->cg_tol = 1e-4f;
+    c->refine_steps = 0;
+    c->m_batches = 1;
+    c->n_batches = 1;
+    c->device = 0;
+    c->batch_work = 0;
+    c->use_graph = 0;
+    return ALS_OK;
+}
+
+int als_config_from_reference(int f, float lambda, int iters, int vtvs, int solve_type, int smem_col_cnt,
+                              int m_batches, int n_batches, als_config *out) {
+    (void)smem_col_cnt;
+    if (!out) return ALS_E_INVALID;
+    als_config_init(out);
+    if (vtvs < 0 || vtvs > 7 || solve_type < 0 || solve_type > 2) return ALS_E_INVALID;
+    out->f = f;
+    out->lambda = lambda;
+    out->iters = iters;
+    out->solve_type = solve_type;
+    out->gram_prec = vtvs >= 5 ? ALS_PREC_FP16 : ALS_PREC_FP32;
+    out->m_batches = m_batches;
+    out->n_batches = n_batches;
+    return ALS_OK;
+}
+
+int als_create(const als_config *cfg, als_engine **out) {
+    if (!cfg || !out) return ALS_E_INVALID;
+    *out = nullptr;
+    if (cfg->f < 1 || blocks_for_T(cfg->f) > MAX_T) return ALS_E_INVALID;
+    if (cfg->gram_prec < ALS_PREC_FP32 || cfg->gram_prec > ALS_PREC_FP16) return ALS_E_INVALID;
+    if (cfg->solve_type < ALS_SOLVE_LU || cfg->solve_type > ALS_SOLVE_CHOL) return ALS_E_INVALID;
+    if (cfg->refine_steps < 0 || cfg->refine_steps > 3 || cfg->cg_iters < 0) return ALS_E_INVALID;
+    if (!(cfg->lambda >= 0.f)) return ALS_E_INVALID;
+    auto *e = new als_engine;
+    e->cfg = *cfg;
+    e->T = blocks_for_T(cfg->f);
+    e->ld = 16 * e->T;
+    e->prec = cfg->gram_prec;
+    e->solve = (cfg->solve_type == ALS_SOLVE_CG_FP32 || cfg->solve_type == ALS_SOLVE_CG_FP16) ? SOLVE_CG : SOLVE_CHOL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || cfg->device < 0 || cfg->device >= ndev) {
+        delete e;
+        return ALS_E_HIP;
+    }
+    if (hipSetDevice(cfg->device) != hipSuccess) { delete e; return ALS_E_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { delete e; return ALS_E_HIP; }
+    e->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&e->own_stream, hipStreamNonBlocking) != hipSuccess) { delete e; return ALS_E_HIP; }
+    e->stream = e->own_stream;
+    e->lds = update_lds_bytes(e->T, e->prec, e->solve, e->solve == SOLVE_CHOL ? cfg->refine_steps : 0);
+    const void *k = update_kernel_ptr(e->T, e->prec, e->solve);
+    int rc = ALS_OK;
+    if (!k || e->lds <= 0 || e->lds > (int)prop.sharedMemPerBlock) {
+        if (e->lds > 65536 && k)
+            (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, e->lds);
+        if (!k || e->lds > 163840) rc = set_err(e, ALS_E_UNSUPPORTED, "kernel variant needs too much LDS");
+    }
+    if (!rc && e->lds > 65536 &&
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, e->lds) != hipSuccess)
+        rc = set_err(e, ALS_E_UNSUPPORTED, "cannot raise the LDS limit for this kernel variant");
+    int per_cu = 0;
+    if (!rc && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, e->lds) != hipSuccess)
+        rc = set_err(e, ALS_E_HIP, "occupancy query failed");
+    if (!rc) {
+        per_cu = std::max(1, per_cu);
+        e->side[0].grid = e->side[1].grid = per_cu * e->num_cu;
+    }
+    if (!rc) rc = dev_alloc(e, &e->queue, 2);
+    if (!rc) rc = dev_alloc(e, &e->fail, 1);
+    if (!rc) rc = dev_alloc(e, &e->partials, e->rmse_blocks);
+    if (!rc) rc = dev_alloc(e, &e->dsum, 1);
+    if (!rc) {
+        if (hipMemsetAsync(e->queue, 0, 2 * sizeof(int), e->stream) != hipSuccess ||
+            hipMemsetAsync(e->fail, 0, sizeof(int), e->stream) != hipSuccess ||
+            hipStreamSynchronize(e->stream) != hipSuccess)
+            rc = set_err(e, ALS_E_HIP, "initial clear failed");
+    }
+    if (rc) {
+        als_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return ALS_OK;
+}
+
+void als_destroy(als_engine *e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    for (void *p : e->allocs) (void)hipFree(p);
+    if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
+    delete e;
+}
+
+const char *als_last_error(const als_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int als_set_stream(als_engine *e, void *s) {
+    if (!e) return ALS_E_INVALID;
+    drop_graph(e);
+    e->stream = s ? static_cast<hipStream_t>(s) : e->own_stream;
+    return ALS_OK;
+}
+
+int als_set_train(als_engine *e, int64_t m, int64_t n, int64_t nnz, const int *csr_ptr, const int *csr_idx,
+                  const float *csr_val, const int *csc_ptr, const int *csc_idx, const float *csc_val) {
+    if (!e) return ALS_E_INVALID;
+    if (m < 0 || n < 0 || nnz < 0 || nnz >= (int64_t)INT32_MAX || m >= (int64_t)INT32_MAX || n >= (int64_t)INT32_MAX)
+        return set_err(e, ALS_E_INVALID, "sizes must be non-negative and fit int32 indices");
+    if (!csr_ptr || !csc_ptr || (nnz > 0 && (!csr_idx || !csr_val || !csc_idx || !csc_val)))
+        return set_err(e, ALS_E_INVALID, "null train array");
+    int rc = check_csr(e, m, n, nnz, csr_ptr, csr_idx, "CSR");
+    if (!rc) rc = check_csr(e, n, m, nnz, csc_ptr, csc_idx, "CSC");
+    if (rc) return rc;
+    if (e->have_train) return set_err(e, ALS_E_STATE, "train data already set");
+    drop_graph(e);
+    e->m = m;
+    e->n = n;
+    e->nnz = nnz;
+    Side &su = e->side[0], &si = e->side[1];
+    su.rows = m;
+    si.rows = n;
+    if ((rc = dev_alloc(e, &su.ptr, m + 1)) || (rc = dev_alloc(e, &su.idx, nnz)) || (rc = dev_alloc(e, &su.val, nnz)) ||
+        (rc = dev_alloc(e, &si.ptr, n + 1)) || (rc = dev_alloc(e, &si.idx, nnz)) || (rc = dev_alloc(e, &si.val, nnz)) ||
+        (rc = dev_alloc(e, &su.order, m)) || (rc = dev_alloc(e, &si.order, n)) || (rc = dev_alloc(e, &e->coo_row, nnz)) ||
+        (rc = dev_alloc(e, &e->U, m * e->ld)) || (rc = dev_alloc(e, &e->V, n * e->ld)))
+        return rc;
+    if (e->prec != PREC_FP32 && ((rc = dev_alloc(e, &e->Us, m * e->ld)) || (rc = dev_alloc(e, &e->Vs, n * e->ld))))
+        return rc;
+    HIPCHK(e, hipMemcpyAsync(su.ptr, csr_ptr, (m + 1) * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(si.ptr, csc_ptr, (n + 1) * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    if (nnz > 0) {
+        HIPCHK(e, hipMemcpyAsync(su.idx, csr_idx, nnz * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(su.val, csr_val, nnz * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(si.idx, csc_idx, nnz * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(si.val, csc_val, nnz * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        std::vector<int> rows(nnz);
+        for (int64_t r = 0; r < m; r++)
+            for (int64_t k = csr_ptr[r]; k < csr_ptr[r + 1]; k++) rows[k] = (int)r;
+        HIPCHK(e, hipMemcpyAsync(e->coo_row, rows.data(), nnz * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
+    HIPCHK(e, hipMemsetAsync(e->U, 0, (size_t)std::max<int64_t>(1, m * e->ld) * sizeof(float), e->stream));
+    HIPCHK(e, hipMemsetAsync(e->V, 0, (size_t)std::max<int64_t>(1, n * e->ld) * sizeof(float), e->stream));
+    if (e->Us) HIPCHK(e, hipMemsetAsync(e->Us, 0, (size_t)std::max<int64_t>(1, m * e->ld) * 2, e->stream));
+    if (e->Vs) HIPCHK(e, hipMemsetAsync(e->Vs, 0, (size_t)std::max<int64_t>(1, n * e->ld) * 2, e->stream));
+    if ((rc = build_side(e, su, m, csr_ptr)) || (rc = build_side(e, si, n, csc_ptr))) return rc;
+    e->have_train = true;
+    return ALS_OK;
+}
+
+int als_set_test(als_engine *e, int64_t n_test, const int *row, const int *col, const float *val) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (n_test < 0 || n_test >= (int64_t)INT32_MAX) return set_err(e, ALS_E_INVALID, "bad test size");
+    if (n_test > 0 && (!row || !col || !val)) return set_err(e, ALS_E_INVALID, "null test array");
+    for (int64_t t = 0; t < n_test; t++)
+        if (row[t] < 0 || row[t] >= e->m || col[t] < 0 || col[t] >= e->n)
+            return set_err(e, ALS_E_INVALID, "test index out of range");
+    if (e->t_row) { dev_free(e, e->t_row); dev_free(e, e->t_col); dev_free(e, e->t_val); }
+    int rc;
+    if ((rc = dev_alloc(e, &e->t_row, n_test)) || (rc = dev_alloc(e, &e->t_col, n_test)) ||
+        (rc = dev_alloc(e, &e->t_val, n_test)))
+        return rc;
+    if (n_test > 0) {
+        HIPCHK(e, hipMemcpyAsync(e->t_row, row, n_test * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(e->t_col, col, n_test * sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(e->t_val, val, n_test * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
+    e->n_test = n_test;
+    e->have_test = true;
+    return ALS_OK;
+}
+
+// Reference: als_model.cu:1772-1777. glibc's srand(0)/rand() sequence is
+// reproduced with the reentrant random_r on a private TYPE_3 state (the
+// state size srand uses), so the process-wide rand() state is untouched.
+int als_init_factors(als_engine *e, int mode) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    const int f = e->cfg.f;
+    std::vector<float> Vh((size_t)e->n * e->ld, 0.f);
+    if (mode == ALS_INIT_REFERENCE) {
+        struct random_data rd;
+        char state[128];
+        std::memset(&rd, 0, sizeof(rd));
+        std::memset(state, 0, sizeof(state));
+        initstate_r(0u, state, sizeof(state), &rd);
+        for (int64_t k = 0; k < e->n * f; k++) {
+            int32_t r = 0;
+            random_r(&rd, &r);
+            Vh[(size_t)(k / f) * e->ld + (k % f)] = (float)(0.2 * ((float)r / (float)RAND_MAX));
+        }
+    } else if (mode != ALS_INIT_ZERO) {
+        return set_err(e, ALS_E_INVALID, "unknown init mode");
+    }
+    HIPCHK(e, hipMemcpyAsync(e->V, Vh.data(), Vh.size() * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->U, 0, (size_t)std::max<int64_t>(1, e->m * e->ld) * sizeof(float), e->stream));
+    int rc = refresh_shadow(e, 0);
+    if (!rc) rc = refresh_shadow(e, 1);
+    if (rc) return rc;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ALS_OK;
+}
+
+int als_set_factors(als_engine *e, int sd, const float *host, int64_t rows, int f) {
+    if (!e || !host) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (f != e->cfg.f || rows != (sd == 0 ? e->m : e->n)) return set_err(e, ALS_E_INVALID, "shape mismatch");
+    float *dst = sd == 0 ? e->U : e->V;
+    HIPCHK(e, hipMemsetAsync(dst, 0, (size_t)std::max<int64_t>(1, rows * e->ld) * sizeof(float), e->stream));
+    if (rows > 0)
+        HIPCHK(e, hipMemcpy2DAsync(dst, e->ld * sizeof(float), host, f * sizeof(float), f * sizeof(float), rows,
+                                   hipMemcpyHostToDevice, e->stream));
+    int rc = refresh_shadow(e, sd);
+    if (rc) return rc;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ALS_OK;
+}
+
+int als_get_factors(als_engine *e, int sd, float *host, int64_t rows, int f) {
+    if (!e || !host) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (f != e->cfg.f || rows != (sd == 0 ? e->m : e->n)) return set_err(e, ALS_E_INVALID, "shape mismatch");
+    const float *src = sd == 0 ? e->U : e->V;
+    if (rows > 0)
+        HIPCHK(e, hipMemcpy2DAsync(host, f * sizeof(float), src, e->ld * sizeof(float), f * sizeof(float), rows,
+                                   hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ALS_OK;
+}
+
+int als_device_factors(als_engine *e, int sd, float **d) {
+    if (!e || !d || (sd != 0 && sd != 1)) return ALS_E_INVALID;
+    *d = sd == 0 ? e->U : e->V;
+    return ALS_OK;
+}
+
+int als_update(als_engine *e, int sd) {
+    if (!e) return ALS_E_INVALID;
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    return update_side(e, sd);
+}
+
+int als_sweep(als_engine *e) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "als_set_train has not been called");
+    if (!e->cfg.use_graph) return sweep_direct(e);
+    if (!e->graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = sweep_direct(e);
+        hipError_t st = hipStreamEndCapture(e->stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (st != hipSuccess) return set_err(e, ALS_E_HIP, std::string("graph capture: ") + hipGetErrorString(st));
+        st = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (st != hipSuccess) { e->graph = nullptr; return set_err(e, ALS_E_HIP, "graph instantiate failed"); }
+    }
+    HIPCHK(e, hipGraphLaunch(e->graph, e->stream));
+    return ALS_OK;
+}
+
+int als_train(als_engine *e, int sweeps, double *rmse) {
+    if (!e) return ALS_E_INVALID;
+    if (sweeps < 0) sweeps = e->cfg.iters;
+    for (int s = 0; s < sweeps; s++) {
+        int rc = als_sweep(e);
+        if (rc) return rc;
+        if (rmse) {
+            rc = als_rmse(e, ALS_RMSE_TEST, &rmse[s]);
+            if (rc) return rc;
+        }
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ALS_OK;
+}
+
+int als_rmse(als_engine *e, int which, double *out) {
+    if (!e || !out) return ALS_E_INVALID;
+    if (which != ALS_RMSE_TEST && which != ALS_RMSE_TRAIN) return set_err(e, ALS_E_INVALID, "bad rmse set");
+    return rmse_launch(e, which, out);
+}
+
+int als_synchronize(als_engine *e) {
+    if (!e) return ALS_E_INVALID;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return ALS_OK;
+}
+
+int als_get_info(als_engine *e, als_info *o) {
+    if (!e || !o) return ALS_E_INVALID;
+    std::memset(o, 0, sizeof(*o));
+    o->T = e->T;
+    o->ld = e->ld;
+    o->grid[0] = e->side[0].grid;
+    o->grid[1] = e->side[1].grid;
+    o->lds_bytes = e->lds;
+    o->nbatches[0] = e->side[0].nbatches;
+    o->nbatches[1] = e->side[1].nbatches;
+    int f = 0;
+    HIPCHK(e, hipMemcpyAsync(&f, e->fail, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    o->failures = f;
+    o->m = e->m;
+    o->n = e->n;
+    o->nnz = e->nnz;
+    o->n_test = e->n_test;
+    return ALS_OK;
+}
+
+// Reference: als_model::save_libmf (als_model.cu:2294-2312), the cumf_sgd /
+// libmf binary model: int loss=0, int m, int n, int f, float mean rating,
+// then U (m x f) and V (n x f) as float32. The mean is summed in FP64 here
+// (the reference uses cublasSasum in FP32).
+int als_save_libmf(als_engine *e, const char *path) {
+    if (!e || !path) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    const int f = e->cfg.f;
+    std::vector<float> U((size_t)e->m * f), V((size_t)e->n * f), vals(e->nnz);
+    int rc = als_get_factors(e, 0, U.data(), e->m, f);
+    if (!rc) rc = als_get_factors(e, 1, V.data(), e->n, f);
+    if (rc) return rc;
+    if (e->nnz > 0) {
+        HIPCHK(e, hipMemcpyAsync(vals.data(), e->side[0].val, e->nnz * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
+    double s = 0.0;
+    for (float v : vals) s += std::fabs((double)v);
+    float mean = e->nnz > 0 ? (float)(s / (double)e->nnz) : 0.f;
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return set_err(e, ALS_E_IO, std::string(path) + ": cannot open");
+    int hdr[4] = {0, (int)e->m, (int)e->n, f};
+    out.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    out.write(reinterpret_cast<const char *>(&mean), sizeof(mean));
+    out.write(reinterpret_cast<const char *>(U.data()), U.size() * sizeof(float));
+    out.write(reinterpret_cast<const char *>(V.data()), V.size() * sizeof(float));
+    if (!out) return set_err(e, ALS_E_IO, std::string(path) + ": write failed");
+    return ALS_OK;
+}
+
+int als_forward_dev(als_engine *e, int sd, const float *d_Y, float *d_X, const float *d_lambda_rows, int save) {
+    if (!e || !d_Y || !d_X) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (e->solve != SOLVE_CHOL) return set_err(e, ALS_E_UNSUPPORTED, "the adjoint needs the Cholesky solver");
+    const int64_t rows = sd == 0 ? e->m : e->n, cols = sd == 0 ? e->n : e->m;
+    int rc = ALS_OK;
+    const uint16_t *Ys = nullptr;
+    if (e->prec != PREC_FP32) {
+        size_t need = (size_t)std::max<int64_t>(1, cols * e->ld) * 2;
+        if (e->tmp16_bytes < need) {
+            dev_free(e, e->tmp16);
+            if ((rc = dev_alloc(e, &e->tmp16, need / 2))) return rc;
+            e->tmp16_bytes = need;
+        }
+        HIPCHK(e, launch_to16(e->prec, d_Y, e->tmp16, cols * e->ld, e->stream));
+        Ys = e->tmp16;
+    }
+    float *Ls = nullptr;
+    if (save) {
+        const size_t np = (size_t)e->ld * (e->ld + 1) / 2;
+        size_t need = (size_t)std::max<int64_t>(1, rows) * np * sizeof(float);
+        if (e->Lsave_bytes < need) {
+            dev_free(e, e->Lsave);
+            if ((rc = dev_alloc(e, &e->Lsave, need / sizeof(float)))) return rc;
+            e->Lsave_bytes = need;
+        }
+        Ls = e->Lsave;
+    }
+    return launch_side(e, sd, d_Y, Ys, d_X, nullptr, d_lambda_rows, Ls);
+}
+
+int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, const float *d_gbar, float *d_xbar,
+                    float *d_lambda_bar, float *d_ybar) {
+    if (!e || !d_Y || !d_X || !d_gbar || !d_xbar || !d_lambda_bar) return ALS_E_INVALID;
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    const int64_t rows = sd == 0 ? e->m : e->n, cols = sd == 0 ? e->n : e->m;
+    const size_t np = (size_t)e->ld * (e->ld + 1) / 2;
+    if (!e->Lsave || e->Lsave_bytes < (size_t)rows * np * sizeof(float))
+        return set_err(e, ALS_E_STATE, "run als_forward_dev with save_factor first");
+    AdjointArgs a{};
+    a.ptr = e->side[sd].ptr;
+    a.X = d_X;
+    a.Lsave = e->Lsave;
+    a.Gbar = d_gbar;
+    a.Xbar = d_xbar;
+    a.lambda_bar = d_lambda_bar;
+    a.rows = (int)rows;
+    a.f = e->cfg.f;
+    if (rows > 0) HIPCHK(e, launch_adjoint_solve(e->T, a, e->stream));
+    if (d_ybar && cols > 0) {
+        YbarArgs y{};
+        const Side &other = e->side[1 - sd];  // CSC of this side's matrix = CSR of the other side
+        y.cptr = other.ptr;
+        y.cidx = other.idx;
+        y.cval = other.val;
+        y.X = d_X;
+        y.Xbar = d_xbar;
+        y.Y = d_Y;
+        y.Ybar = d_ybar;
+        y.ncols = (int)cols;
+        y.f = e->cfg.f;
+        HIPCHK(e, launch_ybar(e->T, y, e->stream));
+    }
+    return ALS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+// Kernels around the per-row update: test/train RMSE with an FP64 sum,
+// FP32 -> 16-bit factor shadows, and the two adjoint kernels; plus the
+// dispatch from (T, precision, solver) to the per-precision update objects.
+#include "als_device.hpp"
+
+namespace als {
+
+const void *update_kernel_ptr_p0(int T, int solve);
+const void *update_kernel_ptr_p1(int T, int solve);
+const void *update_kernel_ptr_p2(int T, int solve);
+int update_lds_bytes_p0(int T, int solve, int refine);
+int update_lds_bytes_p1(int T, int solve, int refine);
+int update_lds_bytes_p2(int T, int solve, int refine);
+hipError_t launch_update_p0(int T, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s);
+hipError_t launch_update_p1(int T, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s);
+hipError_t launch_update_p2(int T, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s);
+
+const void *update_kernel_ptr(int T, int prec, int solve) {
+    switch (prec) {
+    case PREC_FP32: return update_kernel_ptr_p0(T, solve);
+    case PREC_BF16: return update_kernel_ptr_p1(T, solve);
+    case PREC_FP16: return update_kernel_ptr_p2(T, solve);
+    default: return nullptr;
+    }
+}
+
+int update_lds_bytes(int T, int prec, int solve, int refine) {
+    switch (prec) {
+    case PREC_FP32: return update_lds_bytes_p0(T, solve, refine);
+    case PREC_BF16: return update_lds_bytes_p1(T, solve, refine);
+    case PREC_FP16: return update_lds_bytes_p2(T, solve, refine);
+    default: return -1;
+    }
+}
+
+hipError_t launch_update(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s) {
+    switch (prec) {
+    case PREC_FP32: return launch_update_p0(T, solve, a, grid, lds, s);
+    case PREC_BF16: return launch_update_p1(T, solve, a, grid, lds, s);
+    case PREC_FP16: return launch_update_p2(T, solve, a, grid, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------------ RMSE
+// Reference: calculate_square_error (als_model.cu:62-89) + cublasSasum +
+// host sqrt (:1744-1764). Sixteen lanes share one rating: each loads float4
+// slices of u and v, the FP32 dot is reduced across the 16 lanes, and the
+// squared error is accumulated in FP64. Every block writes one partial; a
+// second kernel sums the partials in a fixed order, so the result is
+// deterministic (no float atomics).
+constexpr int RMSE_THREADS = 256;
+
+__global__ __launch_bounds__(RMSE_THREADS) void rmse_kernel(RmseArgs a) {
+    __shared__ double red[RMSE_THREADS / WAVE];
+    const int sub = threadIdx.x & 15;
+    const int64_t group = ((int64_t)blockIdx.x * RMSE_THREADS + threadIdx.x) >> 4;
+    const int64_t ngroups = (int64_t)gridDim.x * (RMSE_THREADS / 16);
+    double acc = 0.0;
+    for (int64_t t = group; t < a.n; t += ngroups) {
+        const float *u = a.U + (int64_t)a.row[t] * a.ld;
+        const float *v = a.V + (int64_t)a.col[t] * a.ld;
+        float d = 0.f;
+        for (int j = 4 * sub; j < a.f; j += 64) {
+            const f32x4 x = *reinterpret_cast<const f32x4 *>(u + j);
+            const f32x4 y = *reinterpret_cast<const f32x4 *>(v + j);
+            d += x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+        }
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) d += __shfl_xor(d, m, 16);
+        if (sub == 0) {
+            const double e = (double)a.val[t] - (double)d;
+            acc += e * e;
+        }
+    }
+    acc = wave_sum_d(acc);
+    if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < RMSE_THREADS / WAVE; ++w) s += red[w];
+        a.partials[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(WAVE) void sum_partials_kernel(const double *p, int nb, double *out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += WAVE) s += p[i];
+    s = wave_sum_d(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, hipStream_t s) {
+    rmse_kernel<<<blocks, RMSE_THREADS, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    sum_partials_kernel<<<1, WAVE, 0, s>>>(a.partials, blocks, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- 16-bit shadow
+// The reference converts the fixed factors to FP16 before every tensor-core
+// Gram (float2half_array, als_model.cu:54-60, 1883-1889). Here the update
+// kernel refreshes the shadow as it writes each row; this kernel builds it
+// after factors are set from the host.
+template <int E>
+__global__ void to16_kernel(const float *src, uint16_t *dst, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = to16<E>(src[i]);
+}
+
+hipError_t launch_to16(int prec, const float *src, uint16_t *dst, int64_t count, hipStream_t s) {
+    const int blocks = (int)std::min<int64_t>((count + 255) / 256, 65536);
+    if (count == 0) return hipSuccess;
+    if (prec == PREC_BF16) to16_kernel<PREC_BF16><<<blocks, 256, 0, s>>>(src, dst, count);
+    else if (prec == PREC_FP16) to16_kernel<PREC_FP16><<<blocks, 256, 0, s>>>(src, dst, count);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------- adjoint
+// No reference counterpart. For x_u = A_u^-1 b_u and an upstream gradient
+// g_u = dL/dx_u, with the Cholesky factor saved by the forward pass:
+//   xbar_u = A_u^-1 g_u  (forward + back substitution on the saved factor)
+//   dL/dlambda_u = -n_u xbar_u . x_u
+template <int T>
+__global__ __launch_bounds__(WAVE) void adjoint_solve_kernel(AdjointArgs a) {
+    constexpr int FP = 16 * T, NP = FP * (FP + 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) float P[];
+    const int l = lane(), f = a.f;
+    for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
+        const int n = a.ptr[row + 1] - a.ptr[row];
+        const int64_t o = (int64_t)row * FP;
+        float glo = 0.f, ghi = 0.f, lb = 0.f;
+        if (n > 0) {
+            const float *src = a.Lsave + (int64_t)row * NP;
+            for (int e = l; e < NP; e += WAVE) P[e] = src[e];
+            __syncthreads();
+            glo = l < f ? a.Gbar[o + l] : 0.f;
+            ghi = WAVE + l < f ? a.Gbar[o + WAVE + l] : 0.f;
+            solve_l(P, f, glo, ghi);
+            solve_lt(P, f, glo, ghi);
+            const float xlo = l < f ? a.X[o + l] : 0.f;
+            const float xhi = WAVE + l < f ? a.X[o + WAVE + l] : 0.f;
+            lb = -(float)n * wave_sum(glo * xlo + ghi * xhi);
+            __syncthreads();
+        }
+        if (l < FP) a.Xbar[o + l] = l < f ? glo : 0.f;
+        if (WAVE + l < FP) a.Xbar[o + WAVE + l] = WAVE + l < f ? ghi : 0.f;
+        if (l == 0) a.lambda_bar[row] = lb;
+    }
+}
+
+// dL/dy_i = sum_{u in Omega(i)} r_ui xbar_u - xbar_u (x_u . y_i) - x_u (xbar_u . y_i)
+// One wave per column i of the rating matrix, walking its CSC list, so every
+// output row has exactly one writer (no atomics).
+template <int T>
+__global__ __launch_bounds__(WAVE) void ybar_kernel(YbarArgs a) {
+    constexpr int FP = 16 * T;
+    const int l = lane(), f = a.f;
+    for (int col = blockIdx.x; col < a.ncols; col += gridDim.x) {
+        const int64_t oy = (int64_t)col * FP;
+        const float ylo = l < f ? a.Y[oy + l] : 0.f;
+        const float yhi = WAVE + l < f ? a.Y[oy + WAVE + l] : 0.f;
+        float acl = 0.f, ach = 0.f;
+        for (int k = a.cptr[col]; k < a.cptr[col + 1]; ++k) {
+            const int64_t ox = (int64_t)a.cidx[k] * FP;
+            const float r = a.cval[k];
+            const float xl = l < f ? a.X[ox + l] : 0.f, xh = WAVE + l < f ? a.X[ox + WAVE + l] : 0.f;
+            const float bl = l < f ? a.Xbar[ox + l] : 0.f, bh = WAVE + l < f ? a.Xbar[ox + WAVE + l] : 0.f;
+            const float xy = wave_sum(xl * ylo + xh * yhi);
+            const float by = wave_sum(bl * ylo + bh * yhi);
+            acl += r * bl - bl * xy - xl * by;
+            ach += r * bh - bh * xy - xh * by;
+        }
+        if (l < FP) a.Ybar[oy + l] = l < f ? acl : 0.f;
+        if (WAVE + l < FP) a.Ybar[oy + WAVE + l] = WAVE + l < f ? ach : 0.f;
+    }
+}
+
+hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s) {
+    const int FP = 16 * T, NP = FP * (FP + 1) / 2;
+    const int grid = std::max(1, std::min(a.rows, 65536));
+    const size_t lds = (size_t)NP * 4;
+    switch (T) {
+#define ALS_ADJ(TT) case TT: adjoint_solve_kernel<TT><<<grid, WAVE, lds, s>>>(a); break;
+    ALS_ADJ(1) ALS_ADJ(2) ALS_ADJ(3) ALS_ADJ(4) ALS_ADJ(5) ALS_ADJ(6) ALS_ADJ(7) ALS_ADJ(8) ALS_ADJ(9)
+#undef ALS_ADJ
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ybar(int T, const YbarArgs &a, hipStream_t s) {
+    const int grid = std::max(1, std::min(a.ncols, 65536));
+    switch (T) {
+#define ALS_YB(TT) case TT: ybar_kernel<TT><<<grid, WAVE, 0, s>>>(a); break;
+    ALS_YB(1) ALS_YB(2) ALS_YB(3) ALS_YB(4) ALS_YB(5) ALS_YB(6) ALS_YB(7) ALS_YB(8) ALS_YB(9)
+#undef ALS_YB
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace als

@@ -1,0 +1,76 @@
+// Kernel argument blocks and launchers shared by the engine (host) and the
+// device code. Internal to the library: the public boundary is include/als.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace als {
+
+enum Prec { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2 };
+enum SolveKind { SOLVE_CHOL = 0, SOLVE_CG = 1 };
+constexpr int MAX_T = 9;  // augmented dimension f + 1 <= 16 * MAX_T
+
+struct UpdateArgs {
+    const int *ptr;             // CSR of the side being updated (int32, reference layout)
+    const int *idx;
+    const float *val;
+    const float *Y;             // fixed factors, FP32, row stride 16*T
+    const uint16_t *Ys;         // fixed factors, 16-bit shadow, row stride 16*T (16-bit Gram only)
+    float *X;                   // updated factors, FP32, row stride 16*T (CG: warm start in)
+    uint16_t *Xs;               // 16-bit shadow of X to refresh (nullable)
+    const float *lambda_rows;   // per-row lambda (nullable -> lambda)
+    float lambda;
+    int f;
+    const int *order;           // rows, heaviest first
+    const int *batch_begin;     // batch b = order[batch_begin[b] .. batch_begin[b+1])
+    int nbatches;
+    int *queue;                 // [0] next batch, [1] waves that left
+    int cg_iters;
+    float cg_tol;
+    int cg_fp16_a;              // CG on an FP16-rounded A
+    int refine;                 // FP64-residual refinement steps (Cholesky)
+    int *fail_count;            // rows whose solve broke down (x set to 0)
+    float *Lsave;               // packed Cholesky factor per row (nullable)
+};
+
+struct AdjointArgs {
+    const int *ptr;             // CSR of the side that was updated
+    const float *X;             // forward solution, stride 16*T
+    const float *Lsave;         // packed factors from the forward pass
+    const float *Gbar;          // dL/dX, stride 16*T
+    float *Xbar;                // A^-1 Gbar, stride 16*T
+    float *lambda_bar;          // -n_u Xbar_u . X_u
+    int rows;
+    int f;
+};
+
+struct YbarArgs {
+    const int *cptr;            // CSC of the same matrix (columns = fixed-factor rows)
+    const int *cidx;
+    const float *cval;
+    const float *X, *Xbar, *Y;  // stride 16*T
+    float *Ybar;                // stride 16*T (overwritten)
+    int ncols;
+    int f;
+};
+
+struct RmseArgs {
+    const int *row, *col;
+    const float *val;
+    const float *U, *V;         // stride ld
+    int64_t n;
+    int f, ld;
+    double *partials;           // one per block
+};
+
+// Launchers (als_kernels_*.hip). Each returns the hipError_t of the launch.
+int update_lds_bytes(int T, int prec, int solve, int refine);
+hipError_t launch_update(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds_bytes,
+                         hipStream_t s);
+const void *update_kernel_ptr(int T, int prec, int solve);
+hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s);
+hipError_t launch_ybar(int T, const YbarArgs &a, hipStream_t s);
+hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, hipStream_t s);
+hipError_t launch_to16(int prec, const float *src, uint16_t *dst, int64_t count, hipStream_t s);
+
+}  // namespace als

@@ -1,0 +1,63 @@
+// Instantiates the per-row update kernel for one Gram precision (ALS_PREC is
+// set by the build: 0 FP32, 1 BF16, 2 FP16), all tile counts T = 1..9 and
+// both solvers. Compiled once per precision so the three build in parallel.
+#include "als_device.hpp"
+
+#ifndef ALS_PREC
+#error "ALS_PREC must be defined"
+#endif
+
+namespace als {
+
+template <int S>
+static const void *kernel_for(int T) {
+    switch (T) {
+    case 1: return (const void *)&update_kernel<1, ALS_PREC, S>;
+    case 2: return (const void *)&update_kernel<2, ALS_PREC, S>;
+    case 3: return (const void *)&update_kernel<3, ALS_PREC, S>;
+    case 4: return (const void *)&update_kernel<4, ALS_PREC, S>;
+    case 5: return (const void *)&update_kernel<5, ALS_PREC, S>;
+    case 6: return (const void *)&update_kernel<6, ALS_PREC, S>;
+    case 7: return (const void *)&update_kernel<7, ALS_PREC, S>;
+    case 8: return (const void *)&update_kernel<8, ALS_PREC, S>;
+    case 9: return (const void *)&update_kernel<9, ALS_PREC, S>;
+    default: return nullptr;
+    }
+}
+
+template <int T>
+static int lds_for(int solve, int refine) {
+    return Geo<T, ALS_PREC>::lds_bytes(solve == SOLVE_CHOL && refine > 0);
+}
+
+#define ALS_CAT2(a, b) a##b
+#define ALS_CAT(a, b) ALS_CAT2(a, b)
+
+const void *ALS_CAT(update_kernel_ptr_p, ALS_PREC)(int T, int solve) {
+    return solve == SOLVE_CHOL ? kernel_for<SOLVE_CHOL>(T) : kernel_for<SOLVE_CG>(T);
+}
+
+int ALS_CAT(update_lds_bytes_p, ALS_PREC)(int T, int solve, int refine) {
+    switch (T) {
+    case 1: return lds_for<1>(solve, refine);
+    case 2: return lds_for<2>(solve, refine);
+    case 3: return lds_for<3>(solve, refine);
+    case 4: return lds_for<4>(solve, refine);
+    case 5: return lds_for<5>(solve, refine);
+    case 6: return lds_for<6>(solve, refine);
+    case 7: return lds_for<7>(solve, refine);
+    case 8: return lds_for<8>(solve, refine);
+    case 9: return lds_for<9>(solve, refine);
+    default: return -1;
+    }
+}
+
+hipError_t ALS_CAT(launch_update_p, ALS_PREC)(int T, int solve, const UpdateArgs &a, int grid, int lds,
+                                             hipStream_t s) {
+    const void *k = ALS_CAT(update_kernel_ptr_p, ALS_PREC)(T, solve);
+    if (!k) return hipErrorInvalidValue;
+    void *args[] = {const_cast<UpdateArgs *>(&a)};
+    return hipLaunchKernel(k, dim3(grid), dim3(WAVE), args, (size_t)lds, s);
+}
+
+}  // namespace als

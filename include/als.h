@@ -1,0 +1,141 @@
+/*
+ * C ABI of the MI355X ALS engine: a drop-in for the reference's als_model
+ * update path (vserdyuk/cuda_als). Plain pointers and sizes only; no
+ * exceptions cross this boundary; every call returns ALS_OK (0) or a
+ * negative ALS_E* code (als_status.h) and als_last_error() gives the text.
+ *
+ * Threading: one engine per host thread. Every device operation is enqueued
+ * on the engine's stream (als_set_stream) and calls that return host data
+ * synchronise that stream.
+ *
+ * Reference interface replaced by each entry point (file:line in the
+ * reference repository):
+ *   als_config_from_reference  INI keys + enums, main.cpp:28-47,
+ *                              als_model.h:12-27
+ *   als_create / als_set_train / als_set_test
+ *                              als_model::als_model, als_model.h:29-32,
+ *                              als_model.cu:1517-1552 (copies CSR/CSC/COO to
+ *                              the device, cuda_sparse_matrix.cpp:76-110)
+ *   als_destroy                als_model::~als_model, als_model.cu:1554-1568
+ *   als_init_factors           init in als_model::train, als_model.cu:1772-1784
+ *   als_update                 the update-U block (als_model.cu:1797-2029) or
+ *                              the update-V block (:2030-2267): RHS
+ *                              (csrmm2+geam :1816-1852), Gram (:1891-1982),
+ *                              solve (LU_solve_U/V :1570-1720 or
+ *                              updateXWithCGHost, cumf_als_cg/cg.h:30-32)
+ *   als_sweep / als_train      the sweep loop of als_model::train,
+ *                              als_model.cu:1790-2275 (+ CALC_ITER_RMSE :2269)
+ *   als_rmse                   als_model::rsme_test / rsme_train,
+ *                              als_model.cu:1722-1764
+ *   als_get_factors            h_UT / h_VT after train(), als_model.cu:2283-2284
+ *   als_save_libmf             als_model::save_libmf, als_model.cu:2294-2312
+ *   als_forward_dev / als_adjoint_dev
+ *                              no reference counterpart: differentiable
+ *                              half-sweep (adjoint of the batched solve)
+ */
+#ifndef ALS_H
+#define ALS_H
+
+#include <stdint.h>
+
+#include "als_status.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct als_engine als_engine;
+
+/* Solver ids 0..2 keep the reference's SOLVE_TYPE values (als_model.h:23-27).
+ * LU and CHOL solve the same SPD system exactly (Cholesky is used for both);
+ * CG_FP16 runs the CG recurrence on an A rounded to FP16 storage (what the
+ * reference's CUMF_ALS_CG_FP16 intends; its own kernel reinterprets FP32
+ * bits as half, cumf_als_cg/cg.cu:642-646, als_model.cu:2003). */
+enum { ALS_SOLVE_LU = 0, ALS_SOLVE_CG_FP16 = 1, ALS_SOLVE_CG_FP32 = 2, ALS_SOLVE_CHOL = 3 };
+/* Precision the Gram accumulation multiplies in (accumulation is FP32). */
+enum { ALS_PREC_FP32 = 0, ALS_PREC_BF16 = 1, ALS_PREC_FP16 = 2 };
+enum { ALS_SIDE_USERS = 0, ALS_SIDE_ITEMS = 1 };
+enum { ALS_RMSE_TEST = 0, ALS_RMSE_TRAIN = 1 };
+enum { ALS_INIT_REFERENCE = 0, ALS_INIT_ZERO = 1 };
+
+typedef struct {
+    int f;               /* factor rank, 1..143 */
+    float lambda;        /* ALS-WR regularisation, scaled by the row's nnz */
+    int iters;           /* sweeps run by als_train when sweeps < 0 */
+    int solve_type;      /* ALS_SOLVE_* */
+    int gram_prec;       /* ALS_PREC_* */
+    int cg_iters;        /* CG iteration cap (reference CG_ITER = 6) */
+    float cg_tol;        /* CG exit threshold on r.r (reference CG_ERROR = 1e-4) */
+    int refine_steps;    /* FP64-residual refinement steps after Cholesky, 0..3 */
+    int m_batches;       /* accepted for config compatibility; the engine never */
+    int n_batches;       /*   materialises Gram matrices, so no batching is needed */
+    int device;          /* HIP device ordinal */
+    int batch_work;      /* ratings per work-queue batch; 0 = chosen by the scheduler */
+    int use_graph;       /* replay each sweep from a captured hipGraph */
+} als_config;
+
+typedef struct {
+    int T;               /* 16-column blocks of the augmented system */
+    int ld;              /* row stride (floats) of the device factor matrices */
+    int grid[2];         /* persistent waves launched per half-sweep (users, items) */
+    int lds_bytes;       /* LDS per wave */
+    int nbatches[2];     /* work-queue batches per half-sweep */
+    int64_t failures;    /* rows whose factorisation broke down (x set to 0) */
+    int64_t m, n, nnz, n_test;
+} als_info;
+
+int als_config_init(als_config *cfg);
+/* Maps the reference's INI/constructor parameters (main.cpp:28-47,
+ * als_model.h:29-32). CALCULATE_VTVS_TYPE 0..4 (FP32 Gram variants) map to
+ * ALS_PREC_FP32, 5..7 (FP16 tensor-core variants) to ALS_PREC_FP16;
+ * smem_col_cnt is accepted and unused. */
+int als_config_from_reference(int f, float lambda, int iters, int calculate_vtvs_type,
+                              int solve_type, int smem_col_cnt, int m_batches, int n_batches,
+                              als_config *out);
+
+int als_create(const als_config *cfg, als_engine **out);
+void als_destroy(als_engine *e);
+const char *als_last_error(const als_engine *e);
+/* hip_stream: a hipStream_t (NULL selects the engine's own stream). */
+int als_set_stream(als_engine *e, void *hip_stream);
+
+/* Host arrays in the reference layout (cuda_sparse_matrix.h:29-37); copied. */
+int als_set_train(als_engine *e, int64_t m, int64_t n, int64_t nnz,
+                  const int *csr_ptr, const int *csr_idx, const float *csr_val,
+                  const int *csc_ptr, const int *csc_idx, const float *csc_val);
+int als_set_test(als_engine *e, int64_t n_test, const int *row, const int *col, const float *val);
+
+int als_init_factors(als_engine *e, int mode);
+/* Host factor matrices, rows x f row-major (the reference's UT/VT layout). */
+int als_set_factors(als_engine *e, int side, const float *host, int64_t rows, int f);
+int als_get_factors(als_engine *e, int side, float *host, int64_t rows, int f);
+
+int als_update(als_engine *e, int side);
+int als_sweep(als_engine *e);
+/* sweeps < 0 runs cfg.iters; rmse_per_sweep (nullable) receives the test
+ * RMSE after every sweep. */
+int als_train(als_engine *e, int sweeps, double *rmse_per_sweep);
+int als_rmse(als_engine *e, int which, double *out);
+int als_synchronize(als_engine *e);
+int als_get_info(als_engine *e, als_info *out);
+int als_save_libmf(als_engine *e, const char *path);
+
+/* Device factor matrices of the engine (row stride als_info.ld floats). */
+int als_device_factors(als_engine *e, int side, float **d_ptr);
+
+/* Differentiable half-sweep on caller device buffers (row stride ld):
+ *   als_forward_dev: X = solve(A(lambda_rows, Y), b(Y)) for every row of
+ *     `side`; with save_factor the Cholesky factors are kept for the adjoint.
+ *   als_adjoint_dev: given gbar = dL/dX, writes xbar = A^-1 gbar,
+ *     lambda_bar = -n_u xbar_u . x_u, and ybar (the fixed-factor gradient
+ *     through A and b; d_ybar rows x ld is overwritten).
+ * Requires solve_type LU or CHOL. d_lambda_rows may be NULL (scalar lambda). */
+int als_forward_dev(als_engine *e, int side, const float *d_Y, float *d_X,
+                    const float *d_lambda_rows, int save_factor);
+int als_adjoint_dev(als_engine *e, int side, const float *d_Y, const float *d_X,
+                    const float *d_gbar, float *d_xbar, float *d_lambda_bar, float *d_ybar);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

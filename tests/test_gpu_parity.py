@@ -1,0 +1,200 @@
+"""GPU parity: the HIP engine (through its C ABI) against the FP64 oracle on
+identical seeded inputs.
+
+Tolerances (absolute):
+  FP32 path: per-sweep test RMSE and factors within 1e-4 of the oracle.
+  Mixed precision (16-bit Gram, FP32 Cholesky + FP64-residual refinement):
+    per-sweep test RMSE within 2e-4 of the exact FP64 oracle (FP16 Gram);
+    BF16 Gram: within 1e-4 of the oracle run on BF16-rounded factors (its
+    distance to the exact oracle is the format's own error and is reported).
+"""
+import numpy as np
+import pytest
+
+import cuda_als_amd as ca
+from cuda_als_amd import engine as E
+from oracle import als_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(d, f, solve, prec, sweeps, refine=0, lam=0.05, graph=False, batch_work=0):
+    cfg = E.make_config(f, lam, sweeps, solve, prec, refine=refine, use_graph=graph, batch_work=batch_work)
+    with E.Engine(cfg, d) as e:
+        e.init_factors()
+        r = e.train(sweeps)
+        U, V, info = e.get_factors(0), e.get_factors(1), e.info()
+    return U, V, r, info
+
+
+def run_oracle(d, f, solver, sweeps, prec=O.PREC_EXACT, lam=0.05):
+    U0, V0 = O.reference_init(d.m, d.n, f)
+    cfg = O.config(f, lam, solver, gram_prec=prec, store_fp32=True)
+    U, V, r, bad = O.train(cfg, d, sweeps, U0, V0)
+    assert bad == 0
+    return U, V, r
+
+
+def test_reference_init_on_device(toy):
+    f = 16
+    cfg = E.make_config(f)
+    with E.Engine(cfg, toy) as e:
+        e.init_factors()
+        U, V = e.get_factors(0), e.get_factors(1)
+    U0, V0 = O.reference_init(toy.m, toy.n, f)
+    assert np.array_equal(V, V0.astype(np.float32))
+    assert not U.any()
+
+
+@pytest.mark.parametrize("solve", [E.SolveType.CHOL, E.SolveType.LU])
+def test_fp32_cholesky_matches_oracle(toy, solve):
+    U, V, r, info = run_gpu(toy, 16, solve, E.Prec.FP32, 5)
+    Uo, Vo, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
+    print("gpu", r, "\noracle", ro)
+    assert info.failures == 0
+    assert np.max(np.abs(r - ro)) <= 1e-4
+    assert np.max(np.abs(U - Uo)) <= 1e-4
+    assert np.max(np.abs(V - Vo)) <= 1e-4
+
+
+def test_fp16_mixed_matches_exact_oracle(toy):
+    U, V, r, info = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.FP16, 5, refine=2)
+    _, _, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
+    _, _, re = run_oracle(toy, 16, O.SOLVE_CHOL, 5, prec=O.PREC_FP16)
+    print("gpu", r, "\nexact", ro, "\nfp16-rounded oracle", re)
+    assert np.max(np.abs(r - ro)) <= 2e-4
+    assert np.max(np.abs(r - re)) <= 1e-4
+
+
+def test_bf16_half_sweep_matches_rounded_oracle(toy):
+    """Same BF16-rounded inputs on both sides: the kernel must agree with the
+    FP64 solve of the BF16-formed system to FP32 accuracy."""
+    f = 16
+    U0, V0 = O.reference_init(toy.m, toy.n, f)
+    with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.BF16, refine=2), toy) as e:
+        e.init_factors()
+        e.update(ca.ALS_SIDE_USERS)
+        U = e.get_factors(0)
+    Uo, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=O.PREC_BF16), toy.csr_ptr, toy.csr_idx,
+                     toy.csr_val, V0, U0)
+    assert np.max(np.abs(U - Uo)) <= 1e-5 * max(1.0, np.abs(Uo).max())
+
+
+def test_bf16_mixed_trajectory_error_is_reported(toy):
+    """Over sweeps, BF16 rounding flips amplify; the trajectory's distance to
+    the exact oracle is the format's error (the FP64 oracle fed BF16-rounded
+    factors is itself 2.2e-4 away on this workload). Bound it at 5e-4."""
+    U, V, r, info = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.BF16, 5, refine=2)
+    _, _, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
+    _, _, re = run_oracle(toy, 16, O.SOLVE_CHOL, 5, prec=O.PREC_BF16)
+    print("gpu", r, "\nexact", ro, "\nbf16-rounded oracle", re, "\nbf16 error vs exact", np.abs(r - ro).max())
+    assert info.failures == 0
+    assert np.max(np.abs(r[:3] - re[:3])) <= 1e-5
+    assert np.max(np.abs(r - ro)) <= 5e-4
+
+
+def _cg_half_sweep(d, f, solve, osolve, tol, iters=6):
+    U0, V0 = O.reference_init(d.m, d.n, f)
+    rng = np.random.default_rng(3)
+    U0 = (rng.random((d.m, f)) * 0.1).astype(np.float32).astype(np.float64)   # non-zero warm start
+    with E.Engine(E.make_config(f, 0.05, 1, solve, E.Prec.FP32, cg_iters=iters, cg_tol=tol), d) as e:
+        e.init_factors()
+        e.set_factors(0, U0)
+        e.update(ca.ALS_SIDE_USERS)
+        U = e.get_factors(0)
+    Uo, _ = O.update(O.config(f, 0.05, osolve, cg_iters=iters, cg_tol=tol), d.csr_ptr, d.csr_idx, d.csr_val, V0, U0)
+    return U, Uo
+
+
+@pytest.mark.parametrize("solve,osolve", [(E.SolveType.CUMF_ALS_CG_FP32, O.SOLVE_CG),
+                                          (E.SolveType.CUMF_ALS_CG_FP16, O.SOLVE_CG_FP16A)])
+def test_cg_half_sweep_matches_oracle_without_early_exit(toy, solve, osolve):
+    """With the r.r < tol exit disabled the CG iterate is a smooth function of
+    the inputs: FP32 CG must follow the FP64 recurrence (warm start, 6 steps)."""
+    U, Uo = _cg_half_sweep(toy, 16, solve, osolve, 0.0)
+    err = np.max(np.abs(U - Uo)) / max(1.0, np.abs(Uo).max())
+    print("max rel err", err)
+    assert err <= 1e-4
+
+
+def test_cg_early_exit_flips_are_rare(toy):
+    """With tol = 1e-4 a row whose r.r lands near the threshold can stop one
+    iteration earlier in FP32 than in FP64: only those rows may differ."""
+    U, Uo = _cg_half_sweep(toy, 16, E.SolveType.CUMF_ALS_CG_FP32, O.SOLVE_CG, 1e-4)
+    diff = np.max(np.abs(U - Uo), axis=1)
+    frac = np.mean(diff > 1e-4)
+    print("rows differing > 1e-4:", frac)
+    assert frac <= 0.02
+
+
+def test_cg_fp32_trajectory(toy):
+    U, V, r, info = run_gpu(toy, 16, E.SolveType.CUMF_ALS_CG_FP32, E.Prec.FP32, 5)
+    _, _, ro = run_oracle(toy, 16, O.SOLVE_CG, 5)
+    print("gpu", r, "\noracle", ro)
+    assert info.failures == 0
+    assert np.max(np.abs(r - ro)) <= 2e-3
+
+
+@pytest.mark.parametrize("f", [1, 7, 15, 16, 31, 48, 64, 100, 128])
+@pytest.mark.parametrize("prec", [E.Prec.FP32, E.Prec.FP16])
+def test_half_sweep_every_tile_count(toy, f, prec):
+    """One user half-sweep from the reference init, every 16-block count."""
+    U0, V0 = O.reference_init(toy.m, toy.n, f)
+    cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=1 if prec != E.Prec.FP32 else 0)
+    with E.Engine(cfg, toy) as e:
+        e.init_factors()
+        e.update(ca.ALS_SIDE_USERS)
+        U = e.get_factors(0)
+        assert e.info().failures == 0
+    oprec = O.PREC_EXACT if prec == E.Prec.FP32 else O.PREC_FP16
+    Uo, bad = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec), toy.csr_ptr, toy.csr_idx, toy.csr_val,
+                       V0, U0)
+    err = np.max(np.abs(U - Uo)) / max(1.0, np.max(np.abs(Uo)))
+    print(f, prec, err)
+    assert err <= 1e-4
+
+
+def test_rmse_kernel(toy):
+    rng = np.random.default_rng(1)
+    f = 100
+    U = rng.standard_normal((toy.m, f)).astype(np.float32) * 0.2
+    V = rng.standard_normal((toy.n, f)).astype(np.float32) * 0.2
+    with E.Engine(E.make_config(f), toy) as e:
+        e.set_factors(0, U)
+        e.set_factors(1, V)
+        rt, rr = e.rmse(ca.ALS_RMSE_TEST), e.rmse(ca.ALS_RMSE_TRAIN)
+    assert abs(rt - O.rmse(toy.t_row, toy.t_col, toy.t_val, U, V)) <= 1e-6 * rt
+    rows = toy.train_rows()
+    assert abs(rr - O.rmse(rows, toy.csr_idx, toy.csr_val, U, V)) <= 1e-6 * rr
+
+
+def test_empty_rows_get_zero_factors():
+    # users 3 and 7 have no ratings, item 5 has none either
+    rng = np.random.default_rng(7)
+    m, n = 12, 9
+    dense = (rng.random((m, n)) < 0.5) * rng.integers(1, 6, (m, n))
+    dense[[3, 7], :] = 0
+    dense[:, 5] = 0
+    rows, cols = np.nonzero(dense)
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m))])
+    d = ca.from_arrays(m, n, ptr, cols, dense[rows, cols].astype(np.float32), rows[:3], cols[:3],
+                       dense[rows[:3], cols[:3]].astype(np.float32))
+    U, V, r, info = run_gpu(d, 4, E.SolveType.CHOL, E.Prec.FP32, 3)
+    assert info.failures == 0
+    assert not U[[3, 7]].any() and not V[5].any()
+    Uo, Vo, ro = run_oracle(d, 4, O.SOLVE_CHOL, 3)
+    assert np.max(np.abs(U - Uo)) <= 1e-4 and np.max(np.abs(V - Vo)) <= 1e-4
+
+
+def test_graph_replay_matches_direct(toy):
+    _, _, r1, _ = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.FP16, 4, refine=1)
+    _, _, r2, _ = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.FP16, 4, refine=1, graph=True)
+    assert np.array_equal(r1, r2)
+
+
+def test_schedule_invariance(toy):
+    """Batch size changes which wave takes which row, never the result."""
+    _, _, r1, i1 = run_gpu(toy, 32, E.SolveType.CHOL, E.Prec.FP32, 2, batch_work=1)
+    _, _, r2, i2 = run_gpu(toy, 32, E.SolveType.CHOL, E.Prec.FP32, 2, batch_work=100000)
+    assert i1.nbatches[0] != i2.nbatches[0]
+    assert np.array_equal(r1, r2)
