@@ -1,0 +1,179 @@
+"""Headline benchmark: full ALS sweeps/s at the netflix-shaped workload.
+
+Workload: 480,189 users x 17,770 items, 99,072,112 train ratings (1,408,395
+held out), f = 100, lambda = 0.05, seeded synthetic data (seed 42). A step is
+one full sweep (all users from V, then all items from U) with the data
+already resident in HBM. The mixed-precision path (FP16 Gram on MFMA with
+FP32 accumulation, FP32 Cholesky, FP64-residual refinement) is the measured
+line; the plain FP32 path (FP32 MFMA Gram, FP32 Cholesky) is timed beside it
+for the speed ratio and the test-RMSE comparison.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+
+Prints one JSON line. `roofline` covers the dominant kernel (the per-row
+update kernel of the half-sweep that takes longest), timed with HIP events
+on the engine's stream; `cpu_baseline` is the FP64 oracle on a row sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "ALS full sweeps/s @100M nnz f=100, mixed-prec vs FP32 at equal test RMSE"
+PEAK_FP16_TFLOPS = 2500.0     # MI355X dense FP16/BF16 MFMA
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 matrix
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse_traffic(kernel_tag: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = ROOT / "profiles" / "traffic_update_netflix_shape.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        m = re.match(rf"^traffic_bytes_per_launch\s+{kernel_tag}\s+(\S+)", line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def time_path(torch, E, ca, d, cfg, steps, warmup, tag):
+    """Runs warmup + timed sweeps; returns (seconds, per-half kernel ms, rmse, info)."""
+    stream = torch.cuda.Stream()          # default priority; a real (non-null) stream handle
+    torch.cuda.set_stream(stream)
+    eng = E.Engine(cfg, d)
+    eng.set_stream(stream.cuda_stream)
+    eng.init_factors()
+    for _ in range(warmup):
+        eng.sweep()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ev[s][0].record(stream)
+        eng.update(ca.ALS_SIDE_USERS)
+        ev[s][1].record(stream)
+        eng.update(ca.ALS_SIDE_ITEMS)
+        ev[s][2].record(stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    users_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    items_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    rmse = eng.rmse(ca.ALS_RMSE_TEST)
+    info = eng.info()
+    log(f"[{tag}] {steps} sweeps in {dt:.3f}s -> {steps / dt:.2f} sweeps/s; users {users_ms:.3f} ms, "
+        f"items {items_ms:.3f} ms per half; test RMSE {rmse:.6f}; failures {info.failures}")
+    return eng, dt, users_ms, items_ms, rmse, info
+
+
+def cpu_baseline(d, f, lam, budget_s=20.0):
+    """FP64 oracle (Cholesky) on uniformly sampled rows of both halves,
+    scaled to a full sweep. Returns the dict for the JSON line."""
+    import numpy as np
+    from oracle import als_oracle as O
+    rng = np.random.default_rng(0)
+    V = rng.random((d.n, f)) * 0.2
+    U = rng.random((d.m, f)) * 0.2
+    cfg = O.config(f, lam, O.SOLVE_CHOL)
+    per_row = {}
+    for name, rows, ptr, idx, val, Y in (("users", d.m, d.csr_ptr, d.csr_idx, d.csr_val, V),
+                                          ("items", d.n, d.csc_ptr, d.csc_idx, d.csc_val, U)):
+        k = 64
+        spent, total_rows, total_t = 0.0, 0, 0.0
+        while spent < budget_s / 2 and total_rows < rows:
+            sel = rng.integers(0, rows, size=min(k, rows))
+            t = time.perf_counter()
+            O.update_rows(cfg, sel, ptr, idx, val, Y)
+            el = time.perf_counter() - t
+            total_t += el
+            total_rows += sel.size
+            spent += el
+            k *= 2
+        per_row[name] = (total_t / total_rows, total_rows)
+    sweep_s = per_row["users"][0] * d.m + per_row["items"][0] * d.n
+    return {"value": 1.0 / sweep_s, "unit": "sweeps/s", "cores": int(O.lib.orc_num_threads()), "kind": "port",
+            "sample": (f"FP64 oracle Cholesky update of {per_row['users'][1]} uniformly sampled users and "
+                       f"{per_row['items'][1]} sampled items, per-row time scaled to all {d.m} users + {d.n} items")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--f", type=int, default=100)
+    ap.add_argument("--prec", choices=["fp16", "bf16"], default="fp16")
+    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (debug only)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU row sharding is not built "
+              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import cuda_als_amd as ca
+    from cuda_als_amd import engine as E
+
+    t = time.perf_counter()
+    wl = ca.netflix_shape() if a.scale == 1.0 else ca.scaled(ca.netflix_shape(), a.scale)
+    d = wl.generate()
+    log(f"generated {d.m} x {d.n}, {d.nnz} train / {d.n_test} test ratings in {time.perf_counter() - t:.1f}s")
+    f, lam = a.f, 0.05
+    prec = E.Prec.FP16 if a.prec == "fp16" else E.Prec.BF16
+
+    mixed_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine)
+    eng_m, dt_m, um, im, rmse_m, info_m = time_path(torch, E, ca, d, mixed_cfg, a.steps, a.warmup, "mixed")
+    del eng_m
+    fp32 = None
+    if not a.no_fp32:
+        fp32_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, E.Prec.FP32)
+        eng_f, dt_f, uf, itf, rmse_f, info_f = time_path(torch, E, ca, d, fp32_cfg, a.steps, a.warmup, "fp32")
+        del eng_f
+        fp32 = {"sweeps_per_s": a.steps / dt_f, "users_ms": uf, "items_ms": itf, "test_rmse": rmse_f,
+                "failures": int(info_f.failures)}
+
+    # Dominant kernel: the slower of the two half-sweep launches of the mixed path.
+    half, ms, rows_nnz = ("items", im, d.nnz) if im >= um else ("users", um, d.nnz)
+    gram_flops = float(d.nnz) * f * (f + 1)           # lower triangle incl. diagonal, 2 flop per MAC
+    achieved = gram_flops / (ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP16_TFLOPS, "traffic": parse_traffic(f"update_{half}"),
+                "kernel": f"update_kernel (16-bit Gram + Cholesky), {half} half-sweep",
+                "kernel_ms": ms, "flops_per_launch": gram_flops}
+
+    line = {
+        "metric": METRIC, "value": a.steps / dt_m, "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": dt_m / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": a.prec, "data": "synthetic (seeded, seed 42)",
+        "config": {"workload": f"netflix-shape {d.m}x{d.n}, {d.nnz} train ratings, f={f}, lambda={lam}",
+                   "gram": f"{a.prec} MFMA, fp32 accumulate", "solve": f"fp32 Cholesky + {a.refine} fp64-residual refinement",
+                   "users_ms": um, "items_ms": im, "test_rmse_after_steps": rmse_m,
+                   "failures": int(info_m.failures), "grid": list(info_m.grid), "lds_bytes": info_m.lds_bytes},
+        "fp32_path": fp32,
+        "speedup_vs_fp32": (a.steps / dt_m) / fp32["sweeps_per_s"] if fp32 else None,
+        "test_rmse_delta_vs_fp32": abs(rmse_m - fp32["test_rmse"]) if fp32 else None,
+        "roofline": roofline,
+        "cpu_baseline": None if a.no_cpu else cpu_baseline(d, f, lam),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -26,12 +26,13 @@ class AlsConfig(C.Structure):
     _fields_ = [("f", C.c_int), ("lambda_", C.c_float), ("iters", C.c_int), ("solve_type", C.c_int),
                 ("gram_prec", C.c_int), ("cg_iters", C.c_int), ("cg_tol", C.c_float),
                 ("refine_steps", C.c_int), ("m_batches", C.c_int), ("n_batches", C.c_int),
-                ("device", C.c_int), ("batch_work", C.c_int), ("use_graph", C.c_int)]
+                ("device", C.c_int), ("batch_work", C.c_int), ("split_len", C.c_int), ("use_graph", C.c_int)]
 
 
 class AlsInfo(C.Structure):
     _fields_ = [("T", C.c_int), ("ld", C.c_int), ("grid", C.c_int * 2), ("lds_bytes", C.c_int),
-                ("nbatches", C.c_int * 2), ("failures", C.c_int64), ("m", C.c_int64), ("n", C.c_int64),
+                ("nbatches", C.c_int * 2), ("nunits", C.c_int * 2), ("nsplit_rows", C.c_int * 2),
+                ("split_len", C.c_int64 * 2), ("failures", C.c_int64), ("m", C.c_int64), ("n", C.c_int64),
                 ("nnz", C.c_int64), ("n_test", C.c_int64)]
 
 

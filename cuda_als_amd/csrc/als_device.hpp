@@ -438,12 +438,36 @@ __global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
         if (b >= a.nbatches) break;
         const int u0 = a.batch_begin[b], u1 = a.batch_begin[b + 1];
         for (int u = u0; u < u1; ++u) {
-            const int row = a.order[u];
-            const int64_t rs = a.ptr[row];
-            const int n = a.ptr[row + 1] - (int)rs;
+            const int4 w = a.units[u];
+            const int row = w.x;
+            const int n = a.ptr[row + 1] - a.ptr[row];  // whole row (lambda * n_u)
             float xlo = 0.f, xhi = 0.f;
             if (n > 0) {
-                gram_row<T, E>(a, rs, n, acc, lds);
+                gram_row<T, E>(a, w.y, w.z - w.y, acc, lds);
+                if (w.w >= 0) {
+                    // Split row: publish this part's tiles; the last part to
+                    // arrive sums every part in slot order and solves.
+                    const int4 sp = a.splits[w.w];
+                    const int part = (w.y - a.ptr[row]) / sp.z;
+                    float *mine = a.partial + (int64_t)(sp.x + part) * G::NT * 256;
+#pragma unroll
+                    for (int t = 0; t < G::NT; ++t) *reinterpret_cast<f32x4 *>(mine + t * 256 + 4 * l) = acc[t];
+                    __threadfence();
+                    int ticket = 0;
+                    if (l == 0) ticket = atomicAdd(a.counters + w.w, 1);
+                    ticket = __builtin_amdgcn_readfirstlane(__shfl(ticket, 0, WAVE));
+                    if (ticket != sp.y - 1) continue;
+                    __threadfence();
+#pragma unroll
+                    for (int t = 0; t < G::NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    for (int p = 0; p < sp.y; ++p) {
+                        const float *src = a.partial + (int64_t)(sp.x + p) * G::NT * 256;
+#pragma unroll
+                        for (int t = 0; t < G::NT; ++t)
+                            acc[t] += *reinterpret_cast<const f32x4 *>(src + t * 256 + 4 * l);
+                    }
+                    if (l == 0) atomicExch(a.counters + w.w, 0);
+                }
                 const float lam = a.lambda_rows ? a.lambda_rows[row] : a.lambda;
                 add_diag<T>(acc, f, lam * (float)n);
                 bool ok = true;

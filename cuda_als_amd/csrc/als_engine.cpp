@@ -29,8 +29,11 @@ struct Side {
     int64_t rows = 0;           // rows of the matrix this side updates
     int *ptr = nullptr, *idx = nullptr;
     float *val = nullptr;
-    int *order = nullptr, *batch = nullptr;
-    int nbatches = 0;
+    int4 *units = nullptr, *splits = nullptr;
+    int *batch = nullptr, *counters = nullptr;
+    float *partial = nullptr;
+    int nbatches = 0, nunits = 0, nsplits = 0;
+    int64_t nslots = 0, split_len = 0, batch_work = 0;
     int grid = 0;
     float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
 };
@@ -111,13 +114,26 @@ int check_csr(als_engine *e, int64_t rows, int64_t cols, int64_t nnz, const int 
 }
 
 int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr) {
-    Schedule sc = build_schedule(rows, ptr, e->cfg.batch_work, s.grid, e->prec);
-    HIPCHK(e, hipMemcpyAsync(s.order, sc.order.data(), sc.order.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
-    int rc = dev_alloc(e, &s.batch, sc.batch_begin.size());
-    if (rc) return rc;
+    Schedule sc = build_schedule(rows, ptr, e->cfg.split_len, e->cfg.batch_work, s.grid, e->prec);
+    static_assert(sizeof(Unit) == sizeof(int4) && sizeof(Split) == sizeof(int4), "unit layout");
+    int rc;
+    if ((rc = dev_alloc(e, &s.units, sc.units.size())) || (rc = dev_alloc(e, &s.batch, sc.batch_begin.size())) ||
+        (rc = dev_alloc(e, &s.splits, sc.splits.size())) || (rc = dev_alloc(e, &s.counters, sc.splits.size())) ||
+        (rc = dev_alloc(e, &s.partial, (size_t)sc.nslots * (e->T * (e->T + 1) / 2) * 256)))
+        return rc;
+    if (!sc.units.empty())
+        HIPCHK(e, hipMemcpyAsync(s.units, sc.units.data(), sc.units.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
+    if (!sc.splits.empty())
+        HIPCHK(e, hipMemcpyAsync(s.splits, sc.splits.data(), sc.splits.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
     HIPCHK(e, hipMemcpyAsync(s.batch, sc.batch_begin.data(), sc.batch_begin.size() * sizeof(int),
                              hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemsetAsync(s.counters, 0, std::max<size_t>(1, sc.splits.size()) * sizeof(int), e->stream));
     s.nbatches = (int)sc.batch_begin.size() - 1;
+    s.nunits = (int)sc.units.size();
+    s.nsplits = (int)sc.splits.size();
+    s.nslots = sc.nslots;
+    s.split_len = sc.split_len;
+    s.batch_work = sc.batch_work;
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return ALS_OK;
 }
@@ -146,7 +162,10 @@ int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float
     a.lambda_rows = lambda_rows;
     a.lambda = e->cfg.lambda;
     a.f = e->cfg.f;
-    a.order = s.order;
+    a.units = s.units;
+    a.splits = s.splits;
+    a.counters = s.counters;
+    a.partial = s.partial;
     a.batch_begin = s.batch;
     a.nbatches = s.nbatches;
     a.queue = e->queue;
@@ -234,6 +253,7 @@ int als_config_init(als_config *c) {
     c->n_batches = 1;
     c->device = 0;
     c->batch_work = 0;
+    c->split_len = 0;
     c->use_graph = 0;
     return ALS_OK;
 }
@@ -353,7 +373,7 @@ int als_set_train(als_engine *e, int64_t m, int64_t n, int64_t nnz, const int *c
     si.rows = n;
     if ((rc = dev_alloc(e, &su.ptr, m + 1)) || (rc = dev_alloc(e, &su.idx, nnz)) || (rc = dev_alloc(e, &su.val, nnz)) ||
         (rc = dev_alloc(e, &si.ptr, n + 1)) || (rc = dev_alloc(e, &si.idx, nnz)) || (rc = dev_alloc(e, &si.val, nnz)) ||
-        (rc = dev_alloc(e, &su.order, m)) || (rc = dev_alloc(e, &si.order, n)) || (rc = dev_alloc(e, &e->coo_row, nnz)) ||
+        (rc = dev_alloc(e, &e->coo_row, nnz)) ||
         (rc = dev_alloc(e, &e->U, m * e->ld)) || (rc = dev_alloc(e, &e->V, n * e->ld)))
         return rc;
     if (e->prec != PREC_FP32 && ((rc = dev_alloc(e, &e->Us, m * e->ld)) || (rc = dev_alloc(e, &e->Vs, n * e->ld))))
@@ -530,8 +550,12 @@ int als_get_info(als_engine *e, als_info *o) {
     o->grid[0] = e->side[0].grid;
     o->grid[1] = e->side[1].grid;
     o->lds_bytes = e->lds;
-    o->nbatches[0] = e->side[0].nbatches;
-    o->nbatches[1] = e->side[1].nbatches;
+    for (int sd = 0; sd < 2; sd++) {
+        o->nbatches[sd] = e->side[sd].nbatches;
+        o->nunits[sd] = e->side[sd].nunits;
+        o->nsplit_rows[sd] = e->side[sd].nsplits;
+        o->split_len[sd] = e->side[sd].split_len;
+    }
     int f = 0;
     HIPCHK(e, hipMemcpyAsync(&f, e->fail, sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));

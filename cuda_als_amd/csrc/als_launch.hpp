@@ -21,8 +21,11 @@ struct UpdateArgs {
     const float *lambda_rows;   // per-row lambda (nullable -> lambda)
     float lambda;
     int f;
-    const int *order;           // rows, heaviest first
-    const int *batch_begin;     // batch b = order[batch_begin[b] .. batch_begin[b+1])
+    const int4 *units;          // {row, kb, ke, split} work units, heaviest first
+    const int4 *splits;         // {slot_base, nparts, part_len, -} per split row
+    int *counters;              // arrivals per split row (0 between launches)
+    float *partial;             // partial Gram tiles, NT*256 floats per slot
+    const int *batch_begin;     // batch b = units[batch_begin[b] .. batch_begin[b+1])
     int nbatches;
     int *queue;                 // [0] next batch, [1] waves that left
     int cg_iters;

@@ -9,28 +9,50 @@ namespace als {
 
 int64_t row_solve_cost(int prec) { return prec == PREC_FP32 ? 96 : 1024; }
 
-Schedule build_schedule(int64_t rows, const int *ptr, int batch_work, int grid, int prec) {
+Schedule build_schedule(int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work, int grid, int prec) {
     Schedule s;
-    s.order.resize(rows);
-    std::iota(s.order.begin(), s.order.end(), 0);
-    std::stable_sort(s.order.begin(), s.order.end(), [&](int a, int b) {
-        return (ptr[a + 1] - ptr[a]) > (ptr[b + 1] - ptr[b]);
-    });
     const int64_t solve = row_solve_cost(prec);
     int64_t total = 0;
     for (int64_t r = 0; r < rows; r++) total += (int64_t)(ptr[r + 1] - ptr[r]) + solve;
-    int64_t W = batch_work > 0 ? batch_work : std::max<int64_t>(solve, total / std::max<int64_t>(1, (int64_t)grid * 16));
+    const int64_t waves = std::max<int64_t>(1, grid);
+    // Parts small enough that the heaviest row spreads over many waves, large
+    // enough that the partial-tile traffic stays small next to the gathers.
+    s.split_len = split_len > 0 ? split_len : std::max<int64_t>(2048, total / (waves * 8));
+    s.batch_work = batch_work > 0 ? batch_work : std::max<int64_t>(solve, total / (waves * 16));
+
+    struct Item { Unit u; int64_t cost; };
+    std::vector<Item> items;
+    items.reserve(rows + total / s.split_len + 1);
+    for (int64_t r = 0; r < rows; r++) {
+        const int64_t n = ptr[r + 1] - ptr[r];
+        if (n <= s.split_len) {
+            items.push_back({{(int)r, ptr[r], ptr[r + 1], -1}, n + solve});
+            continue;
+        }
+        const int64_t P = (n + s.split_len - 1) / s.split_len;
+        const int64_t len = (n + P - 1) / P;
+        Split sp{(int)s.nslots, (int)P, (int)len, 0};
+        const int sid = (int)s.splits.size();
+        s.splits.push_back(sp);
+        s.nslots += P;
+        for (int64_t p = 0; p < P; p++) {
+            const int64_t kb = ptr[r] + p * len, ke = std::min<int64_t>(ptr[r + 1], kb + len);
+            items.push_back({{(int)r, (int)kb, (int)ke, sid}, (ke - kb) + (p == P - 1 ? solve : 0)});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+    s.units.resize(items.size());
     s.batch_begin.push_back(0);
     int64_t acc = 0;
-    for (int64_t u = 0; u < rows; u++) {
-        const int r = s.order[u];
-        acc += (int64_t)(ptr[r + 1] - ptr[r]) + solve;
-        if (acc >= W) {
-            s.batch_begin.push_back((int)(u + 1));
+    for (size_t i = 0; i < items.size(); i++) {
+        s.units[i] = items[i].u;
+        acc += items[i].cost;
+        if (acc >= s.batch_work) {
+            s.batch_begin.push_back((int)(i + 1));
             acc = 0;
         }
     }
-    if (s.batch_begin.back() != (int)rows) s.batch_begin.push_back((int)rows);
+    if (s.batch_begin.back() != (int)items.size()) s.batch_begin.push_back((int)items.size());
     return s;
 }
 

@@ -1,28 +1,51 @@
-// Work schedule of one half-sweep: rows in decreasing-nnz order (the heavy
-// rows start first, so the tail of the launch is made of light rows) cut into
-// batches that the persistent waves pop from the device queue.
+// Work schedule of one half-sweep.
 //
 // The reference launches one block per row of a fixed batch in row order
-// (als_model.cu:1891-1982) with no load balancing; rows range from 1 to
-// about 10^4 ratings at the sizes this engine targets.
+// (als_model.cu:1891-1982) with no load balancing; at the sizes this engine
+// targets a row has from 1 to ~5*10^5 ratings. Here:
+//   * a row heavier than `split_len` ratings is cut into parts of near-equal
+//     length; every part is its own work unit, each writes its partial Gram
+//     tiles to scratch and the last part to finish sums the partials in part
+//     order (deterministic) and solves;
+//   * units are ordered by decreasing cost (longest-processing-time first),
+//     so the tail of the launch is made of the lightest rows;
+//   * consecutive units are grouped into batches of about `batch_work`
+//     rating-equivalents; persistent waves pop batches from a device queue.
 #pragma once
 #include <cstdint>
 #include <vector>
 
 namespace als {
 
-struct Schedule {
-    std::vector<int> order;        // row ids, heaviest first (ties by row id)
-    std::vector<int> batch_begin;  // batch b = order[batch_begin[b] .. batch_begin[b+1])
+struct Unit {
+    int row;      // row of the side being updated
+    int kb, ke;   // rating range [kb, ke) inside the row's CSR segment (absolute)
+    int split;    // index into Schedule::splits, or -1 for a whole row
 };
 
-// Cost model of one row in rating-equivalents: its Gram work plus a fixed
-// per-row solve cost that depends on the Gram precision (the solve is FP32
-// either way, the Gram is 16x cheaper on BF16/FP16 MFMA).
+struct Split {
+    int slot_base;  // first partial-Gram slot of this row
+    int nparts;
+    int part_len;   // every part but the last has exactly this many ratings
+    int pad;
+};
+
+struct Schedule {
+    std::vector<Unit> units;
+    std::vector<Split> splits;
+    std::vector<int> batch_begin;  // batch b = units[batch_begin[b] .. batch_begin[b+1])
+    int64_t nslots = 0;            // partial-Gram slots needed
+    int64_t split_len = 0;
+    int64_t batch_work = 0;
+};
+
+// Cost of a solve in rating-equivalents of Gram work for the given Gram
+// precision (the solve is FP32 either way; the 16-bit Gram is ~16x cheaper
+// per rating than the FP32 one).
 int64_t row_solve_cost(int prec);
 
-// batch_work <= 0 picks a size that gives each persistent wave about
-// 16 batches; a row heavier than the batch size forms a batch on its own.
-Schedule build_schedule(int64_t rows, const int *ptr, int batch_work, int grid, int prec);
+// split_len <= 0 and batch_work <= 0 select sizes from the total work and
+// the number of persistent waves (`grid`).
+Schedule build_schedule(int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work, int grid, int prec);
 
 }  // namespace als

@@ -71,6 +71,7 @@ typedef struct {
     int n_batches;       /*   materialises Gram matrices, so no batching is needed */
     int device;          /* HIP device ordinal */
     int batch_work;      /* ratings per work-queue batch; 0 = chosen by the scheduler */
+    int split_len;       /* rows with more ratings are split across waves; 0 = chosen */
     int use_graph;       /* replay each sweep from a captured hipGraph */
 } als_config;
 
@@ -80,6 +81,9 @@ typedef struct {
     int grid[2];         /* persistent waves launched per half-sweep (users, items) */
     int lds_bytes;       /* LDS per wave */
     int nbatches[2];     /* work-queue batches per half-sweep */
+    int nunits[2];       /* work units (whole rows + parts of split rows) */
+    int nsplit_rows[2];  /* rows split across waves */
+    int64_t split_len[2];/* ratings per part of a split row */
     int64_t failures;    /* rows whose factorisation broke down (x set to 0) */
     int64_t m, n, nnz, n_test;
 } als_info;

@@ -198,3 +198,54 @@ def test_schedule_invariance(toy):
     _, _, r2, i2 = run_gpu(toy, 32, E.SolveType.CHOL, E.Prec.FP32, 2, batch_work=100000)
     assert i1.nbatches[0] != i2.nbatches[0]
     assert np.array_equal(r1, r2)
+
+
+def _run_split(d, f, prec, split_len, sweeps=2):
+    cfg = E.make_config(f, 0.05, sweeps, E.SolveType.CHOL, prec, refine=1 if prec != E.Prec.FP32 else 0,
+                        split_len=split_len)
+    with E.Engine(cfg, d) as e:
+        e.init_factors()
+        r = e.train(sweeps)
+        return e.get_factors(0), e.get_factors(1), r, e.info()
+
+
+def test_split_rows_match_whole_rows_fp32(toy):
+    """Rows cut into parts (partial Gram tiles summed by the last part) give
+    the whole-row result up to FP32 summation order."""
+    U1, V1, r1, i1 = _run_split(toy, 32, E.Prec.FP32, 8)
+    U2, V2, r2, i2 = _run_split(toy, 32, E.Prec.FP32, 1 << 30)
+    print("split rows", i1.nsplit_rows[0], i1.nsplit_rows[1], "units", i1.nunits[0], i1.nunits[1])
+    assert i1.nsplit_rows[1] > 100 and i2.nsplit_rows[1] == 0
+    assert i1.failures == 0
+    assert np.max(np.abs(U1 - U2)) <= 1e-4 and np.max(np.abs(V1 - V2)) <= 1e-4
+    assert np.max(np.abs(r1 - r2)) <= 1e-5
+
+
+@pytest.mark.parametrize("side", [ca.ALS_SIDE_USERS, ca.ALS_SIDE_ITEMS])
+def test_split_rows_match_whole_rows_fp16_half_sweep(toy, side):
+    """FP16 Gram, identical 16-bit inputs: split and whole rows differ only by
+    the FP32 order of the partial sums. (Across sweeps that order difference
+    can flip an FP16 rounding of a factor, so trajectories are not compared.)"""
+    rng = np.random.default_rng(5)
+    f = 32
+    U0 = (rng.random((toy.m, f)) * 0.3).astype(np.float32)
+    V0 = (rng.random((toy.n, f)) * 0.3).astype(np.float32)
+    out = []
+    for split_len in (8, 1 << 30):
+        cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.FP16, refine=1, split_len=split_len)
+        with E.Engine(cfg, toy) as e:
+            e.set_factors(0, U0)
+            e.set_factors(1, V0)
+            e.update(side)
+            out.append(e.get_factors(side))
+            assert e.info().failures == 0
+    scale = max(1.0, np.abs(out[1]).max())
+    assert np.max(np.abs(out[0] - out[1])) <= 1e-5 * scale
+
+
+def test_split_rows_are_deterministic(toy):
+    """Which part arrives last varies from run to run; the partials are summed
+    in part order, so the result does not."""
+    U1, V1, r1, _ = _run_split(toy, 32, E.Prec.FP16, 5, sweeps=3)
+    U2, V2, r2, _ = _run_split(toy, 32, E.Prec.FP16, 5, sweeps=3)
+    assert np.array_equal(U1, U2) and np.array_equal(V1, V2) and np.array_equal(r1, r2)
