@@ -31,9 +31,13 @@ ORACLE_LIB = ROOT / "oracle" / "libals_oracle.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for every HIP translation unit.
+# MFMA accumulators are kept in VGPRs (not AGPRs): the same tiles feed the
+# VALU solve right after the Gram, and one register pool lets two waves fit
+# per SIMD at f = 100.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     f"-I{INC}", f"-I{CSRC}",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", f"-I{INC}", f"-I{CSRC}"]

@@ -82,15 +82,13 @@ template <> __device__ __forceinline__ float from16<PREC_FP16>(uint16_t h) {
 template <int T, int E> struct Geo {
     static constexpr int NT = T * (T + 1) / 2;  // lower-triangle 16x16 tiles
     static constexpr int FP = 16 * T;           // padded augmented dimension
-    static constexpr int NP = FP * (FP + 1) / 2;// packed lower triangle
     static constexpr int YT_H = 40;             // 16-bit stage row: 32 ratings + 8 pad (80 B)
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
     static constexpr int STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : FP * YT_H * 2;
-    static constexpr int MISC_FLOATS = 32 + 32 + FP + 128;  // ibuf, rbuf, colbuf, vbuf
-    static constexpr int R1_BYTES = (STAGE_BYTES > NP * 4 ? STAGE_BYTES : NP * 4);
-    __host__ __device__ static constexpr int lds_bytes(bool second_packed) {
-        return MISC_FLOATS * 4 + R1_BYTES + (second_packed ? NP * 4 : 0);
-    }
+    static constexpr int PANEL_BYTES = (T - 1) * 16 * 17 * 4;  // Cholesky panel tiles (after the Gram)
+    static constexpr int MISC_FLOATS = 32 + 32 + 128 + 128;  // ibuf, rbuf, colbuf, xbuf
+    static constexpr int R1_BYTES = STAGE_BYTES > PANEL_BYTES ? STAGE_BYTES : PANEL_BYTES;
+    __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + R1_BYTES; }
 };
 
 // ------------------------------------------------------------------ Gram
@@ -129,9 +127,9 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
             __syncthreads();
             int k = l / nseg, s = l % nseg;
             for (int task = l; task < CHUNK * nseg; task += WAVE) {
-                int id = ibuf[k];
-                f32x4 v = {0.f, 0.f, 0.f, 0.f};
-                if (id >= 0) v = *reinterpret_cast<const f32x4 *>(a.Y + (int64_t)id * G::FP + 4 * s);
+                const int id = ibuf[k];
+                const float *src = id >= 0 ? a.Y + (int64_t)id * G::FP + 4 * s : a.zero32 + 4 * s;
+                const f32x4 v = *reinterpret_cast<const f32x4 *>(src);
                 *reinterpret_cast<f32x4 *>(Ys + k * G::YR + 4 * s) = v;
                 k += dk;
                 s += ds;
@@ -177,19 +175,20 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                 u32x4 d[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    int id = ibuf[8 * g + e];
-                    u32x4 z = {0u, 0u, 0u, 0u};
-                    if (id >= 0) z = *reinterpret_cast<const u32x4 *>(Ysh + (int64_t)id * G::FP + 8 * s);
-                    d[e] = z;
+                    // Unused slots of the last chunk read a row of zeros:
+                    // the load itself is unconditional.
+                    const int id = ibuf[8 * g + e];
+                    const uint16_t *src = id >= 0 ? Ysh + (int64_t)id * G::FP + 8 * s : a.zero16 + 8 * s;
+                    d[e] = *reinterpret_cast<const u32x4 *>(src);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     u32x4 lo, hi;
 #pragma unroll
                     for (int w = 0; w < 4; ++w) {
-                        uint32_t x0 = d[2 * w][v], x1 = d[2 * w + 1][v];
-                        lo[w] = (x0 & 0xffffu) | (x1 << 16);
-                        hi[w] = (x0 >> 16) | (x1 & 0xffff0000u);
+                        const uint32_t x0 = d[2 * w][v], x1 = d[2 * w + 1][v];
+                        lo[w] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);  // low halves of x0, x1
+                        hi[w] = __builtin_amdgcn_perm(x1, x0, 0x07060302u);  // high halves
                     }
                     *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v) * G::YT_H + 8 * g) = lo;
                     *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v + 1) * G::YT_H + 8 * g) = hi;
@@ -229,206 +228,504 @@ __device__ __forceinline__ void add_diag(f32x4 (&acc)[NT], int f, float reg) {
 #pragma unroll
     for (int I = 0; I < T; ++I)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            if (4 * q + r == c) {
-                int d = 16 * I + c;
-                if (d < f) acc[tile_id(I, I)][r] += reg;
-                else if (d > f) acc[tile_id(I, I)][r] = 1.f;
-            }
+        for (int r = 0; r < 4; ++r) {
+            const bool on = 4 * q + r == c;
+            const int d = 16 * I + c;
+            float v = acc[tile_id(I, I)][r];
+            v = (on && d < f) ? v + reg : v;
+            v = (on && d > f) ? 1.f : v;
+            acc[tile_id(I, I)][r] = v;
+        }
 }
 
-// Lower triangle of the register tiles -> packed row-major LDS
-// (P[i(i+1)/2 + j], j <= i). With ROUND16 the values pass through FP16
-// storage (the "FP16-stored A" of the reference's CG_FP16 solver).
-template <int T, int NT, bool ROUND16 = false>
-__device__ __forceinline__ void dump_packed(const f32x4 (&acc)[NT], float *P, int f = 0) {
-    const int l = lane(), q = l >> 4, c = l & 15;
+// ------------------------------------------------------- vector forms
+// A vector v of the padded dimension 16T lives in one of two register forms:
+//   column form  vc[J]      = v[16J + c]          (replicated over q)
+//   row form     vr[I][r]   = v[16I + 4q + r]     (replicated over c)
+// matching the columns and the rows a lane owns in the accumulator tiles.
+
+__device__ __forceinline__ f32x4 col_to_row(float vcJ) {
+    const int l = lane(), q = l >> 4;
+    f32x4 o;
 #pragma unroll
-    for (int I = 0; I < T; ++I)
+    for (int r = 0; r < 4; ++r) o[r] = __shfl(vcJ, (l & 48) | (4 * q + r), WAVE);
+    return o;
+}
+
+__device__ __forceinline__ float row_to_col(const f32x4 &vrJ) {
+    const int c = lane() & 15, src = (c >> 2) << 4;
+    const float t0 = __shfl(vrJ[0], src, WAVE), t1 = __shfl(vrJ[1], src, WAVE);
+    const float t2 = __shfl(vrJ[2], src, WAVE), t3 = __shfl(vrJ[3], src, WAVE);
+    const int r = c & 3;
+    return r == 0 ? t0 : r == 1 ? t1 : r == 2 ? t2 : t3;
+}
+
+// Sum over the 16 lanes of a row (the column index c); every lane gets it.
+__device__ __forceinline__ float row16_sum(float v) {
 #pragma unroll
-        for (int J = 0; J <= I; ++J)
+    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+__device__ __forceinline__ double row16_sum_d(double v) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 16 * I + 4 * q + r, j = 16 * J + c;
-                if (J < I || c <= 4 * q + r) {
-                    float v = acc[tile_id(I, J)][r];
-                    if (ROUND16 && i != f) v = (float)(_Float16)v;  // A in FP16, b stays FP32
-                    P[i * (i + 1) / 2 + j] = v;
-                }
-            }
+    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+// Sum over the 4 row groups (q); every lane gets it.
+__device__ __forceinline__ float q4_sum(float v) {
+    v += __shfl_xor(v, 16, WAVE);
+    return v + __shfl_xor(v, 32, WAVE);
+}
+__device__ __forceinline__ double q4_sum_d(double v) {
+    v += __shfl_xor(v, 16, WAVE);
+    return v + __shfl_xor(v, 32, WAVE);
+}
+
+// Row f of the augmented tiles (b before the factorisation, z = L^-1 b
+// after it), as a column-form vector masked to the first f entries. Row f
+// lies in the last block row (f >> 4 == T - 1 for T = ceil((f+1)/16)).
+template <int T, int NT>
+__device__ __forceinline__ void extract_row_f(const f32x4 (&a)[NT], int f, float (&vc)[T]) {
+    const int l = lane(), c = l & 15;
+    const int src = (((f & 15) >> 2) << 4) | c, rf = f & 3;
+#pragma unroll
+    for (int J = 0; J < T; ++J) {
+        const float v = __shfl(pick4(a[tile_id(T - 1, J)], rf), src, WAVE);
+        vc[J] = (16 * J + c < f) ? v : 0.f;
+    }
 }
 
 // ------------------------------------------------------------- Cholesky
-// Right-looking Cholesky of the leading f x f block of the augmented matrix,
-// in place in the accumulator tiles. Column k is published through LDS
-// (colbuf) once per step; every lane then applies the rank-1 update to its
-// 4 rows x 1 column of every trailing tile. Row f of the result is
-// z = L^-1 b. Returns false if a pivot was not positive.
-template <int T, int K, int NT>
-__device__ __forceinline__ void chol_block(f32x4 (&a)[NT], int f, float *colbuf, bool &ok) {
-    if constexpr (K < T) {
-        const int l = lane(), q = l >> 4, c = l & 15;
-        const int kend = min(16, f - 16 * K);
-        for (int cc = 0; cc < kend; ++cc) {
-            const float d = bcast_lane(pick4(a[tile_id(K, K)], cc & 3), (cc >> 2) * 16 + cc);
-            const bool good = d > 0.f;
-            ok = ok && good;
-            const float inv = good ? 1.0f / sqrtf(d) : 0.f;
-            if (c == cc) {
+// Blocked right-looking Cholesky of the leading f x f block of the
+// augmented matrix, in place in the accumulator tiles (16-column panels).
+//  * Panel: for each pivot column k the raw column is published through
+//    LDS (colbuf); the panel's later columns take the rank-1 update
+//    A[i][j] -= A[i][k] A[j][k] / A[k][k]. Columns stay unscaled until the
+//    panel is done, then every lane scales its own column by 1/sqrt(pivot).
+//  * Trailing tiles: A_IJ -= L_IK L_JK^T for the whole panel at once on FP32
+//    MFMA (16x16x4), the panel tiles passing through LDS to get the MFMA
+//    operand layout.
+// Row f of the result is z = L^-1 b. ok becomes false if a pivot was <= 0.
+template <int T, int K, int R, int NT>
+__device__ __forceinline__ void chol_step(f32x4 (&a)[NT], int cc4, int kend, float *colbuf, bool &ok,
+                                          float &myinv) {
+    const int cc = cc4 + R;  // cc & 3 == R: static register selection below
+    if (cc >= kend) return;
+    const int l = lane(), q = l >> 4, c = l & 15;
+    const float d = bcast_lane(a[tile_id(K, K)][R], (cc4 << 2) | cc);
+    const bool good = d > 0.f;
+    ok = ok && good;
+    const float rs = good ? __builtin_amdgcn_rsqf(d) : 0.f;
+    if (c == cc) {
+        myinv = rs;
 #pragma unroll
-                for (int I = K; I < T; ++I) {
-                    a[tile_id(I, K)] *= inv;
-                    f32x4 v = a[tile_id(I, K)];
-                    if (I == K) {
+        for (int I = K; I < T; ++I) {
+            f32x4 v = a[tile_id(I, K)];
+            if (I == K) {
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (4 * q + r < cc) v[r] = 0.f;
-                    }
-                    *reinterpret_cast<f32x4 *>(colbuf + 16 * I + 4 * q) = v;
-                }
+                for (int r = 0; r < 4; ++r)
+                    if (4 * q + r < cc) v[r] = 0.f;
             }
-            __syncthreads();
-            f32x4 lrow[T - K];
-            float lcol[T - K];
-#pragma unroll
-            for (int I = K; I < T; ++I) {
-                lrow[I - K] = *reinterpret_cast<const f32x4 *>(colbuf + 16 * I + 4 * q);
-                lcol[I - K] = colbuf[16 * I + c];
-            }
-            if (c <= cc) lcol[0] = 0.f;
-#pragma unroll
-            for (int I = K; I < T; ++I)
-#pragma unroll
-                for (int J = K; J <= I; ++J) a[tile_id(I, J)] -= lrow[I - K] * lcol[J - K];
-            __syncthreads();
+            *reinterpret_cast<f32x4 *>(colbuf + 16 * I + 4 * q) = v;
         }
-        chol_block<T, K + 1>(a, f, colbuf, ok);
+    }
+    __syncthreads();
+    const float lk = (c > cc) ? colbuf[16 * K + c] * (rs * rs) : 0.f;
+#pragma unroll
+    for (int I = K; I < T; ++I) a[tile_id(I, K)] -= *reinterpret_cast<const f32x4 *>(colbuf + 16 * I + 4 * q) * lk;
+    __syncthreads();
+}
+
+template <int T, int K, int NT>
+__device__ __forceinline__ void chol_panel(f32x4 (&a)[NT], int f, float *colbuf, float *panel, bool &ok) {
+    if constexpr (K < T) {
+        if (16 * K < f) {
+            const int l = lane(), q = l >> 4, c = l & 15;
+            const int kend = min(16, f - 16 * K);
+            float myinv = 1.f;
+            for (int cc4 = 0; cc4 < kend; cc4 += 4) {
+                chol_step<T, K, 0>(a, cc4, kend, colbuf, ok, myinv);
+                chol_step<T, K, 1>(a, cc4, kend, colbuf, ok, myinv);
+                chol_step<T, K, 2>(a, cc4, kend, colbuf, ok, myinv);
+                chol_step<T, K, 3>(a, cc4, kend, colbuf, ok, myinv);
+            }
+#pragma unroll
+            for (int I = K; I < T; ++I) a[tile_id(I, K)] *= myinv;
+            if constexpr (K + 1 < T) {
+                constexpr int TS = 16 * 17;  // one panel tile, rows padded to 17
+#pragma unroll
+                for (int I = K + 1; I < T; ++I)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) panel[(I - K - 1) * TS + (4 * q + r) * 17 + c] = a[tile_id(I, K)][r];
+                __syncthreads();
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    float F[T];
+#pragma unroll
+                    for (int I = K + 1; I < T; ++I) F[I] = panel[(I - K - 1) * TS + (l & 15) * 17 + 4 * s + (l >> 4)];
+#pragma unroll
+                    for (int I = K + 1; I < T; ++I) {
+                        const float nf = -F[I];
+#pragma unroll
+                        for (int J = K + 1; J <= I; ++J)
+                            a[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x4f32(nf, F[J], a[tile_id(I, J)], 0, 0, 0);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        chol_panel<T, K + 1>(a, f, colbuf, panel, ok);
     }
 }
 
 template <int T, int NT>
-__device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, float *colbuf) {
+__device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, float *colbuf, float *panel) {
     bool ok = true;
-    chol_block<T, 0>(a, f, colbuf, ok);
+    chol_panel<T, 0>(a, f, colbuf, panel, ok);
     return ok;
 }
 
-// Triangular solves on a packed lower factor P (row-major, P[i(i+1)/2+j]).
-// Vectors live one element per lane: v_lo = v[lane], v_hi = v[64 + lane].
+// -------------------------------------------------- triangular solves
+// On the factor held in the tiles (L[16I+4q+r][16J+c] in a[tile(I,J)][r]),
+// block by block: the diagonal 16x16 block is solved column by column with
+// cross-lane broadcasts, the off-diagonal blocks enter as tile mat-vecs.
 
-// L^T x = z, in place (z in, x out).
-__device__ __forceinline__ void solve_lt(const float *P, int f, float &vlo, float &vhi) {
-    const int l = lane();
-    for (int k = f - 1; k >= 0; --k) {
-        const int kk = k * (k + 1) / 2;
-        const float zk = k < WAVE ? bcast_lane(vlo, k) : bcast_lane(vhi, k - WAVE);
-        const float xk = zk * __builtin_amdgcn_rcpf(P[kk + k]);
-        vlo = (l == k) ? xk : vlo;
-        vhi = (WAVE + l == k) ? xk : vhi;
-        if (l < k) vlo -= P[kk + l] * xk;
-        if (WAVE + l < k) vhi -= P[kk + WAVE + l] * xk;
+// Diagonal block of L^T x = y (column form in/out): column j, j = jend-1..0.
+template <int R>
+__device__ __forceinline__ void bsub_step(const f32x4 &D, int j4, int jend, float &y, float &xk) {
+    const int j = j4 + R;
+    if (j >= jend) return;
+    const int c = lane() & 15;
+    const float yj = bcast_lane(y, j);
+    const float ljj = bcast_lane(D[R], (j4 << 2) | j);
+    const float xj = yj * __builtin_amdgcn_rcpf(ljj);
+    xk = (c == j) ? xj : xk;
+    const float lji = __shfl(D[R], (j4 << 2) | c, WAVE);  // L[j][c] of this block
+    y = (c < j) ? y - lji * xj : y;
+}
+
+// L^T x = z. v: column form, z in, x out (entries >= f are zero). The
+// solved blocks also go to the wave's LDS vector buffer xbuf (16T floats),
+// where later blocks read them in row form with one 16-byte load; on return
+// xbuf holds x.
+template <int T, int NT>
+__device__ __forceinline__ void back_subst(const f32x4 (&a)[NT], int f, float (&v)[T], float *xbuf) {
+    const int l = lane(), q = l >> 4, c = l & 15;
+    const int KP = (f - 1) >> 4;
+#pragma unroll
+    for (int K = T - 1; K >= 0; --K) {
+        if (K > KP) {
+            v[K] = 0.f;
+            if (q == 0) xbuf[16 * K + c] = 0.f;
+            __syncthreads();
+            continue;
+        }
+        float p = 0.f;
+#pragma unroll
+        for (int I = K + 1; I < T; ++I) {
+            const f32x4 xr = *reinterpret_cast<const f32x4 *>(xbuf + 16 * I + 4 * q);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) p += a[tile_id(I, K)][r] * xr[r];
+        }
+        float y = v[K] - q4_sum(p);
+        const int jend = min(16, f - 16 * K);
+        float xk = 0.f;
+        for (int j4 = (jend - 1) & ~3; j4 >= 0; j4 -= 4) {
+            bsub_step<3>(a[tile_id(K, K)], j4, jend, y, xk);
+            bsub_step<2>(a[tile_id(K, K)], j4, jend, y, xk);
+            bsub_step<1>(a[tile_id(K, K)], j4, jend, y, xk);
+            bsub_step<0>(a[tile_id(K, K)], j4, jend, y, xk);
+        }
+        v[K] = xk;
+        if (q == 0) xbuf[16 * K + c] = xk;
+        __syncthreads();
     }
 }
 
-// L w = r, in place (r in, w out).
-__device__ __forceinline__ void solve_l(const float *P, int f, float &vlo, float &vhi) {
-    const int l = lane();
-    const int il = l, ih = WAVE + l;
-    const int bl = il * (il + 1) / 2, bh = ih * (ih + 1) / 2;
-    for (int k = 0; k < f; ++k) {
-        const int kk = k * (k + 1) / 2;
-        const float rk = k < WAVE ? bcast_lane(vlo, k) : bcast_lane(vhi, k - WAVE);
-        const float wk = rk * __builtin_amdgcn_rcpf(P[kk + k]);
-        vlo = (l == k) ? wk : vlo;
-        vhi = (WAVE + l == k) ? wk : vhi;
-        if (il > k && il < f) vlo -= P[bl + k] * wk;
-        if (ih > k && ih < f) vhi -= P[bh + k] * wk;
+// Diagonal block of L w = y (row form y, w column form): column j = 0..jend-1.
+template <int R>
+__device__ __forceinline__ void fsub_step(const f32x4 &D, int j4, int jend, f32x4 &y, float &wk) {
+    const int j = j4 + R;
+    if (j >= jend) return;
+    const int l = lane(), q = l >> 4, c = l & 15;
+    const float yj = bcast_lane(y[R], j4 << 2);
+    const float ljj = bcast_lane(D[R], (j4 << 2) | j);
+    const float wj = yj * __builtin_amdgcn_rcpf(ljj);
+    wk = (c == j) ? wj : wk;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float lij = __shfl(D[k], (l & 48) | j, WAVE);  // L[4q+k][j] of this block
+        y[k] = (4 * q + k > j) ? y[k] - lij * wj : y[k];
     }
 }
 
-// y = A p for a packed symmetric A (lane-per-row); p passes through vbuf.
-__device__ __forceinline__ void matvec_packed(const float *A, int f, float plo, float phi, float *vbuf,
-                                              float &ylo, float &yhi) {
-    const int l = lane();
-    vbuf[l] = plo;
-    vbuf[WAVE + l] = phi;
-    __syncthreads();
-    const int il = l, ih = WAVE + l;
-    const int bl = il * (il + 1) / 2, bh = ih * (ih + 1) / 2;
-    float sl = 0.f, sh = 0.f;
-    for (int j = 0; j < f; ++j) {
-        const float pj = vbuf[j];
-        const int bj = j * (j + 1) / 2;
-        if (il < f) sl += A[j <= il ? bl + j : bj + il] * pj;
-        if (ih < f) sh += A[j <= ih ? bh + j : bj + ih] * pj;
+// Forward substitution for block K given the block's right-hand side y (row
+// form, already reduced by the earlier blocks). Returns w_K (column form).
+template <int T, int K, int NT>
+__device__ __forceinline__ float fsub_block(const f32x4 (&a)[NT], int f, f32x4 y) {
+    const int jend = min(16, f - 16 * K);
+    float wk = 0.f;
+    for (int j4 = 0; j4 < jend; j4 += 4) {
+        fsub_step<0>(a[tile_id(K, K)], j4, jend, y, wk);
+        fsub_step<1>(a[tile_id(K, K)], j4, jend, y, wk);
+        fsub_step<2>(a[tile_id(K, K)], j4, jend, y, wk);
+        fsub_step<3>(a[tile_id(K, K)], j4, jend, y, wk);
     }
-    __syncthreads();
-    ylo = sl;
-    yhi = sh;
+    return wk;
 }
 
-// r = b - A x in FP64 (b is row f of the packed augmented matrix).
-__device__ __forceinline__ void residual_fp64(const float *A, int f, float xlo, float xhi, float *vbuf,
-                                              double &rlo, double &rhi) {
-    const int l = lane();
-    vbuf[l] = xlo;
-    vbuf[WAVE + l] = xhi;
-    __syncthreads();
-    const int il = l, ih = WAVE + l;
-    const int bl = il * (il + 1) / 2, bh = ih * (ih + 1) / 2, bf = f * (f + 1) / 2;
-    double sl = il < f ? (double)A[bf + il] : 0.0, sh = ih < f ? (double)A[bf + ih] : 0.0;
-    for (int j = 0; j < f; ++j) {
-        const double xj = (double)vbuf[j];
-        const int bj = j * (j + 1) / 2;
-        if (il < f) sl -= (double)A[j <= il ? bl + j : bj + il] * xj;
-        if (ih < f) sh -= (double)A[j <= ih ? bh + j : bj + ih] * xj;
+template <int T, int K, int NT>
+__device__ __forceinline__ void fwd_blocks(const f32x4 (&a)[NT], int f, const f32x4 (&vr)[T], float (&w)[T]) {
+    if constexpr (K < T) {
+        w[K] = 0.f;
+        if (16 * K < f) {
+            f32x4 u = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int J = 0; J < K; ++J) u += a[tile_id(K, J)] * w[J];
+            f32x4 y = vr[K];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) y[r] -= row16_sum(u[r]);
+            w[K] = fsub_block<T, K>(a, f, y);
+        }
+        fwd_blocks<T, K + 1>(a, f, vr, w);
     }
-    __syncthreads();
-    rlo = sl;
-    rhi = sh;
 }
 
-// cuMF CG (cumf_als_cg/cg.cu:37-232): warm start from x, cg_iters
-// iterations, exit when r.r < tol after an update; p.Ap == 0 also stops.
-__device__ __forceinline__ void cg_packed(const float *A, int f, int iters, float tol, float *vbuf,
-                                          float &xlo, float &xhi) {
-    const int l = lane(), bf = f * (f + 1) / 2;
-    const float blo = l < f ? A[bf + l] : 0.f, bhi = WAVE + l < f ? A[bf + WAVE + l] : 0.f;
-    float axl, axh;
-    matvec_packed(A, f, xlo, xhi, vbuf, axl, axh);
-    float rlo = blo - axl, rhi = bhi - axh;
-    float plo = rlo, phi = rhi;
-    float rsold = wave_sum(rlo * rlo + rhi * rhi);
+// L w = r. vr: row form in (entries >= f ignored); w: column form out.
+template <int T, int NT>
+__device__ __forceinline__ void fwd_subst(const f32x4 (&a)[NT], int f, const f32x4 (&vr)[T], float (&w)[T]) {
+    fwd_blocks<T, 0>(a, f, vr, w);
+}
+
+// ------------------------------------------------------------ mat-vecs
+// y = A v for the symmetric matrix held as lower tiles (full diagonal
+// tiles) in the accumulator layout, restricted to the leading f x f block.
+// v in column form (zero past f); y returned in column form (zero past f).
+// Lower tile (I,J), lane (q,c): A[16I+4q+r][16J+c] contributes
+//   to y[16I+4q+r] with v[16J+c]   (row partial, summed over c)
+//   to y[16J+c]   with v[16I+4q+r] (column partial, summed over q; J < I)
+template <int T, int NT>
+__device__ __forceinline__ void matvec_f32(const f32x4 (&A)[NT], int f, const float (&v)[T], float (&y)[T]) {
+    const int c = lane() & 15, q = lane() >> 4;
+    f32x4 vr[T];
+#pragma unroll
+    for (int I = 0; I < T; ++I) vr[I] = col_to_row(v[I]);
+    f32x4 pr[T];
+    float pc[T];
+#pragma unroll
+    for (int I = 0; I < T; ++I) {
+        pr[I] = f32x4{0.f, 0.f, 0.f, 0.f};
+        pc[I] = 0.f;
+    }
+#pragma unroll
+    for (int I = 0; I < T; ++I)
+#pragma unroll
+        for (int J = 0; J <= I; ++J) {
+            const f32x4 t = A[tile_id(I, J)];
+            pr[I] += t * v[J];
+            if (J < I) pc[J] += t[0] * vr[I][0] + t[1] * vr[I][1] + t[2] * vr[I][2] + t[3] * vr[I][3];
+        }
+#pragma unroll
+    for (int I = 0; I < T; ++I) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pr[I][r] = row16_sum(pr[I][r]);
+        const float yc = row_to_col(pr[I]) + q4_sum(pc[I]);
+        y[I] = (16 * I + c < f) ? yc : 0.f;
+    }
+    (void)q;
+}
+
+// r = b - A x with A read back from this wave's global scratch copy of the
+// tiles; products and sums in FP64. x, b in column form; r returned in row
+// form (FP32, zero past f) for the forward substitution.
+template <int T>
+__device__ __forceinline__ void residual_fp64(const float *Ag, int f, const float (&b)[T], const float (&x)[T],
+                                              f32x4 (&r)[T]) {
+    const int l = lane(), q = l >> 4;
+    f32x4 xr[T];
+#pragma unroll
+    for (int I = 0; I < T; ++I) xr[I] = col_to_row(x[I]);
+    // One output block J at a time: y_J = sum_{I>=J} A_IJ^T x_I (column
+    // partials; the diagonal tile is full and symmetric) + sum_{I<J} A_JI x_I
+    // (row partials). Tiles are re-read from the scratch copy (L2-resident).
+#pragma unroll
+    for (int J = 0; J < T; ++J) {
+        double pc = 0.0, pr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int I = J; I < T; ++I) {
+            const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(I, J) * 256 + 4 * l);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) pc += (double)t[k] * (double)xr[I][k];
+        }
+#pragma unroll
+        for (int I = 0; I < J; ++I) {
+            const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(J, I) * 256 + 4 * l);
+            const double xi = (double)x[I];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) pr[k] += (double)t[k] * xi;
+        }
+        const double pcs = q4_sum_d(pc);        // column form: y_J[c]
+        const f32x4 br = col_to_row(b[J]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double colpart = __shfl(pcs, (l & 48) | (4 * q + k), WAVE);  // -> row form
+            const int i = 16 * J + 4 * q + k;
+            const double y = row16_sum_d(pr[k]) + colpart;
+            r[J][k] = i < f ? (float)((double)br[k] - y) : 0.f;
+        }
+    }
+}
+
+// One step of iterative refinement, x += A^-1 (b - A x), with the residual
+// in FP64 and the correction solved on the FP32 factor. The residual of
+// block K is formed just before block K of the forward substitution needs
+// it, so only one block of it is ever live. Ag is this wave's scratch copy
+// of the augmented tiles before the factorisation (row f holds b).
+template <int T, int K, int NT>
+__device__ __forceinline__ void refine_blocks(const f32x4 (&a)[NT], int f, const float *Ag, const float (&x)[T],
+                                              const float *xbuf, float (&w)[T]) {
+    if constexpr (K < T) {
+        w[K] = 0.f;
+        // Keep the scratch-tile loads of this block from being hoisted into
+        // earlier blocks (that would hold every tile of A in registers).
+        __builtin_amdgcn_sched_barrier(0);
+        if (16 * K < f) {
+            const int l = lane(), q = l >> 4;
+            const int bsrc = ((f & 15) >> 2) << 4, brf = f & 3;
+            double pc = 0.0, pr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int I = K; I < T; ++I) {
+                const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(I, K) * 256 + 4 * l);
+                const f32x4 xr = *reinterpret_cast<const f32x4 *>(xbuf + 16 * I + 4 * q);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) pc += (double)t[k] * (double)xr[k];
+            }
+#pragma unroll
+            for (int I = 0; I < K; ++I) {
+                const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(K, I) * 256 + 4 * l);
+                const double xi = (double)x[I];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) pr[k] += (double)t[k] * xi;
+            }
+            const double pcs = q4_sum_d(pc);
+            const float *brow = Ag + tile_id(T - 1, K) * 256;  // row f lies in block T-1
+            f32x4 u = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int J = 0; J < K; ++J) u += a[tile_id(K, J)] * w[J];
+            f32x4 y;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = 16 * K + 4 * q + k;
+                const double colpart = __shfl(pcs, (l & 48) | (4 * q + k), WAVE);
+                const double bk = (double)brow[4 * (bsrc | (4 * q + k)) + brf];
+                const double rs = row16_sum_d(pr[k]);
+                y[k] = (i < f ? (float)(bk - rs - colpart) : 0.f) - row16_sum(u[k]);
+            }
+            w[K] = fsub_block<T, K>(a, f, y);
+        }
+        refine_blocks<T, K + 1>(a, f, Ag, x, xbuf, w);
+    }
+}
+
+template <int T, int NT>
+__device__ __forceinline__ void refine_step(const f32x4 (&a)[NT], int f, const float *Ag, float (&x)[T],
+                                            float *xbuf) {
+    float w[T];
+    refine_blocks<T, 0>(a, f, Ag, x, xbuf, w);
+    back_subst<T>(a, f, w, xbuf);  // xbuf now holds the correction
+    const int q = lane() >> 4, c = lane() & 15;
+#pragma unroll
+    for (int J = 0; J < T; ++J) {
+        x[J] += w[J];
+        if (q == 0) xbuf[16 * J + c] = x[J];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float colform_dot(const float *a, const float *b, int T) {
+    float s = 0.f;
+    for (int J = 0; J < T; ++J) s += a[J] * b[J];
+    return row16_sum(s);
+}
+
+// cuMF CG (cumf_als_cg/cg.cu:37-232) on the register tiles: warm start x,
+// `iters` iterations, exit when r.r < tol after an update; p.Ap == 0 stops.
+template <int T, int NT>
+__device__ __forceinline__ void cg_regs(const f32x4 (&A)[NT], int f, const float (&b)[T], float (&x)[T], int iters,
+                                        float tol) {
+    float r[T], p[T], ap[T];
+    matvec_f32<T>(A, f, x, ap);
+#pragma unroll
+    for (int J = 0; J < T; ++J) {
+        r[J] = b[J] - ap[J];
+        p[J] = r[J];
+    }
+    float rsold = colform_dot(r, r, T);
     for (int it = 0; it < iters; ++it) {
-        float aplo, aphi;
-        matvec_packed(A, f, plo, phi, vbuf, aplo, aphi);
-        const float pap = wave_sum(plo * aplo + phi * aphi);
+        matvec_f32<T>(A, f, p, ap);
+        const float pap = colform_dot(p, ap, T);
         if (pap == 0.f) break;
         const float alpha = rsold / pap;
-        xlo += alpha * plo;
-        xhi += alpha * phi;
-        rlo -= alpha * aplo;
-        rhi -= alpha * aphi;
-        const float rsnew = wave_sum(rlo * rlo + rhi * rhi);
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+            x[J] += alpha * p[J];
+            r[J] -= alpha * ap[J];
+        }
+        const float rsnew = colform_dot(r, r, T);
         if (rsnew < tol) break;
         const float beta = rsnew / rsold;
         rsold = rsnew;
-        plo = rlo + beta * plo;
-        phi = rhi + beta * phi;
+#pragma unroll
+        for (int J = 0; J < T; ++J) p[J] = r[J] + beta * p[J];
     }
+}
+
+// Column-form vector -> one row of a 16T-strided matrix in global memory
+// (row-group q stores blocks q, q+4, ...; 16 consecutive floats each).
+template <int T>
+__device__ __forceinline__ void store_colform(float *dst, const float (&v)[T]) {
+    const int l = lane(), q = l >> 4, c = l & 15;
+#pragma unroll
+    for (int J4 = 0; J4 < T; J4 += 4) {
+        const int J = J4 + q;
+        float val = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (J4 + k < T && q == k) val = v[J4 + k];
+        if (J < T) dst[16 * J + c] = val;
+    }
+}
+template <int T, int E>
+__device__ __forceinline__ void store_colform16(uint16_t *dst, const float (&v)[T]) {
+    const int l = lane(), q = l >> 4, c = l & 15;
+#pragma unroll
+    for (int J4 = 0; J4 < T; J4 += 4) {
+        const int J = J4 + q;
+        float val = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (J4 + k < T && q == k) val = v[J4 + k];
+        if (J < T) dst[16 * J + c] = to16<E>(val);
+    }
+}
+template <int T>
+__device__ __forceinline__ void load_colform(const float *src, int f, float (&v)[T]) {
+    const int c = lane() & 15;
+#pragma unroll
+    for (int J = 0; J < T; ++J) v[J] = (16 * J + c < f) ? src[16 * J + c] : 0.f;
 }
 
 // ------------------------------------------------------------- the kernel
 template <int T, int E, int S>
-__global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
+__device__ __forceinline__ void update_body(const UpdateArgs &a) {
     using G = Geo<T, E>;
     extern __shared__ __attribute__((aligned(16))) char lds[];
     float *colbuf = reinterpret_cast<float *>(lds) + 64;
-    float *vbuf = colbuf + G::FP;
-    float *R1 = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);
-    float *R2 = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4 + G::R1_BYTES);
+    float *xbuf = colbuf + 128;
+    float *stage = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);  // reused as the panel buffer
     const int l = lane();
     const int f = a.f;
+    float *Ascr = a.scratch ? a.scratch + (int64_t)blockIdx.x * G::NT * 256 : nullptr;
     f32x4 acc[G::NT];
 
     for (;;) {
@@ -441,7 +738,9 @@ __global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
             const int4 w = a.units[u];
             const int row = w.x;
             const int n = a.ptr[row + 1] - a.ptr[row];  // whole row (lambda * n_u)
-            float xlo = 0.f, xhi = 0.f;
+            float x[T];
+#pragma unroll
+            for (int J = 0; J < T; ++J) x[J] = 0.f;
             if (n > 0) {
                 gram_row<T, E>(a, w.y, w.z - w.y, acc, lds);
                 if (w.w >= 0) {
@@ -471,53 +770,47 @@ __global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
                 const float lam = a.lambda_rows ? a.lambda_rows[row] : a.lambda;
                 add_diag<T>(acc, f, lam * (float)n);
                 bool ok = true;
-                if constexpr (S == SOLVE_CHOL) {
-                    if (a.refine > 0) dump_packed<T>(acc, R2);
-                    ok = chol_regs<T>(acc, f, colbuf);
-                    dump_packed<T>(acc, R1);
-                    __syncthreads();
-                    const int bf = f * (f + 1) / 2;
-                    xlo = l < f ? R1[bf + l] : 0.f;
-                    xhi = WAVE + l < f ? R1[bf + WAVE + l] : 0.f;
-                    solve_lt(R1, f, xlo, xhi);
-                    for (int s = 0; s < a.refine; ++s) {
-                        double rl, rh;
-                        residual_fp64(R2, f, xlo, xhi, vbuf, rl, rh);
-                        float dl = (float)rl, dh = (float)rh;
-                        solve_l(R1, f, dl, dh);
-                        solve_lt(R1, f, dl, dh);
-                        xlo += dl;
-                        xhi += dh;
+                if constexpr (S == SOLVE_CHOL || S == SOLVE_CHOL_REFINE) {
+                    if constexpr (S == SOLVE_CHOL_REFINE) {
+#pragma unroll
+                        for (int t = 0; t < G::NT; ++t) *reinterpret_cast<f32x4 *>(Ascr + t * 256 + 4 * l) = acc[t];
+                    }
+                    ok = chol_regs<T>(acc, f, colbuf, stage);
+                    extract_row_f<T>(acc, f, x);
+                    back_subst<T>(acc, f, x, xbuf);
+                    if constexpr (S == SOLVE_CHOL_REFINE) {
+                        for (int s = 0; s < a.refine; ++s) refine_step<T>(acc, f, Ascr, x, xbuf);
                     }
                     if (a.Lsave) {
-                        float *dst = a.Lsave + (int64_t)row * G::NP;
-                        for (int e = l; e < G::NP; e += WAVE) dst[e] = R1[e];
+                        float *dst = a.Lsave + (int64_t)row * G::NT * 256;
+#pragma unroll
+                        for (int t = 0; t < G::NT; ++t) *reinterpret_cast<f32x4 *>(dst + t * 256 + 4 * l) = acc[t];
                     }
                 } else {
-                    if (a.cg_fp16_a) dump_packed<T, G::NT, true>(acc, R1, f);
-                    else dump_packed<T, G::NT, false>(acc, R1);
-                    __syncthreads();
-                    xlo = l < f ? a.X[(int64_t)row * G::FP + l] : 0.f;
-                    xhi = WAVE + l < f ? a.X[(int64_t)row * G::FP + WAVE + l] : 0.f;
-                    cg_packed(R1, f, a.cg_iters, a.cg_tol, vbuf, xlo, xhi);
+                    float bvec[T];
+                    extract_row_f<T>(acc, f, bvec);
+                    if (a.cg_fp16_a) {
+#pragma unroll
+                        for (int t = 0; t < G::NT; ++t)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) acc[t][r] = (float)(_Float16)acc[t][r];
+                    }
+                    load_colform<T>(a.X + (int64_t)row * G::FP, f, x);
+                    cg_regs<T>(acc, f, bvec, x, a.cg_iters, a.cg_tol);
                 }
-                const bool bad = !ok || __any(!(isfinite(xlo) && isfinite(xhi)));
+                bool finite = true;
+#pragma unroll
+                for (int J = 0; J < T; ++J) finite = finite && isfinite(x[J]);
+                const bool bad = !ok || __any(!finite);
                 if (bad) {
                     if (l == 0) atomicAdd(a.fail_count, 1);
-                    xlo = 0.f;
-                    xhi = 0.f;
+#pragma unroll
+                    for (int J = 0; J < T; ++J) x[J] = 0.f;
                 }
-                __syncthreads();
             }
-            float *xr = a.X + (int64_t)row * G::FP;
-            if (l < G::FP) xr[l] = l < f ? xlo : 0.f;
-            if (WAVE + l < G::FP) xr[WAVE + l] = WAVE + l < f ? xhi : 0.f;
+            store_colform<T>(a.X + (int64_t)row * G::FP, x);
             if constexpr (E != PREC_FP32) {
-                if (a.Xs) {
-                    uint16_t *xs = a.Xs + (int64_t)row * G::FP;
-                    if (l < G::FP) xs[l] = l < f ? to16<E>(xlo) : (uint16_t)0;
-                    if (WAVE + l < G::FP) xs[WAVE + l] = WAVE + l < f ? to16<E>(xhi) : (uint16_t)0;
-                }
+                if (a.Xs) store_colform16<T, E>(a.Xs + (int64_t)row * G::FP, x);
             }
         }
     }
@@ -530,6 +823,19 @@ __global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
             atomicExch(a.queue + 1, 0);
         }
     }
+}
+
+template <int T, int E, int S>
+__global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
+    update_body<T, E, S>(a);
+}
+
+// The refinement variant holds the factor, the solution and the FP64
+// residual pieces at once; at two waves per SIMD it trades a small spill
+// for twice the latency hiding of the one-wave form.
+template <int T, int E, int S>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) void update_kernel_w2(UpdateArgs a) {
+    update_body<T, E, S>(a);
 }
 
 }  // namespace als

@@ -61,6 +61,9 @@ struct als_engine {
     int rmse_blocks = 1024;
     float *Lsave = nullptr;
     size_t Lsave_bytes = 0;
+    float *scratch = nullptr;          // per-wave A copy for refinement
+    float *zero32 = nullptr;           // one zero row (16T floats) for unused chunk slots
+    uint16_t *zero16 = nullptr;
     uint16_t *tmp16 = nullptr;         // shadow of caller-provided Y (forward_dev)
     size_t tmp16_bytes = 0;
     hipGraphExec_t graph = nullptr;
@@ -172,9 +175,12 @@ int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float
     a.cg_iters = e->cfg.cg_iters;
     a.cg_tol = e->cfg.cg_tol;
     a.cg_fp16_a = e->cfg.solve_type == ALS_SOLVE_CG_FP16;
-    a.refine = e->solve == SOLVE_CHOL ? e->cfg.refine_steps : 0;
+    a.refine = e->solve == SOLVE_CHOL_REFINE ? e->cfg.refine_steps : 0;
     a.fail_count = e->fail;
     a.Lsave = Lsave;
+    a.scratch = e->scratch;
+    a.zero32 = e->zero32;
+    a.zero16 = e->zero16;
     if (s.rows == 0 || s.nbatches == 0) return ALS_OK;
     HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, e->stream));
     return ALS_OK;
@@ -287,7 +293,9 @@ int als_create(const als_config *cfg, als_engine **out) {
     e->T = blocks_for_T(cfg->f);
     e->ld = 16 * e->T;
     e->prec = cfg->gram_prec;
-    e->solve = (cfg->solve_type == ALS_SOLVE_CG_FP32 || cfg->solve_type == ALS_SOLVE_CG_FP16) ? SOLVE_CG : SOLVE_CHOL;
+    e->solve = (cfg->solve_type == ALS_SOLVE_CG_FP32 || cfg->solve_type == ALS_SOLVE_CG_FP16) ? SOLVE_CG
+               : cfg->refine_steps > 0                                                    ? SOLVE_CHOL_REFINE
+                                                                                          : SOLVE_CHOL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || cfg->device < 0 || cfg->device >= ndev) {
         delete e;
@@ -299,7 +307,7 @@ int als_create(const als_config *cfg, als_engine **out) {
     e->num_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&e->own_stream, hipStreamNonBlocking) != hipSuccess) { delete e; return ALS_E_HIP; }
     e->stream = e->own_stream;
-    e->lds = update_lds_bytes(e->T, e->prec, e->solve, e->solve == SOLVE_CHOL ? cfg->refine_steps : 0);
+    e->lds = update_lds_bytes(e->T, e->prec, e->solve, cfg->refine_steps);
     const void *k = update_kernel_ptr(e->T, e->prec, e->solve);
     int rc = ALS_OK;
     if (!k || e->lds <= 0 || e->lds > (int)prop.sharedMemPerBlock) {
@@ -317,13 +325,19 @@ int als_create(const als_config *cfg, als_engine **out) {
         per_cu = std::max(1, per_cu);
         e->side[0].grid = e->side[1].grid = per_cu * e->num_cu;
     }
+    if (!rc && e->solve == SOLVE_CHOL_REFINE)
+        rc = dev_alloc(e, &e->scratch, (size_t)e->side[0].grid * (e->T * (e->T + 1) / 2) * 256);
     if (!rc) rc = dev_alloc(e, &e->queue, 2);
     if (!rc) rc = dev_alloc(e, &e->fail, 1);
     if (!rc) rc = dev_alloc(e, &e->partials, e->rmse_blocks);
     if (!rc) rc = dev_alloc(e, &e->dsum, 1);
+    if (!rc) rc = dev_alloc(e, &e->zero32, e->ld);
+    if (!rc) rc = dev_alloc(e, &e->zero16, e->ld);
     if (!rc) {
         if (hipMemsetAsync(e->queue, 0, 2 * sizeof(int), e->stream) != hipSuccess ||
             hipMemsetAsync(e->fail, 0, sizeof(int), e->stream) != hipSuccess ||
+            hipMemsetAsync(e->zero32, 0, e->ld * sizeof(float), e->stream) != hipSuccess ||
+            hipMemsetAsync(e->zero16, 0, e->ld * sizeof(uint16_t), e->stream) != hipSuccess ||
             hipStreamSynchronize(e->stream) != hipSuccess)
             rc = set_err(e, ALS_E_HIP, "initial clear failed");
     }
@@ -601,7 +615,7 @@ int als_forward_dev(als_engine *e, int sd, const float *d_Y, float *d_X, const f
     if (!e || !d_Y || !d_X) return ALS_E_INVALID;
     if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
     if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
-    if (e->solve != SOLVE_CHOL) return set_err(e, ALS_E_UNSUPPORTED, "the adjoint needs the Cholesky solver");
+    if (e->solve == SOLVE_CG) return set_err(e, ALS_E_UNSUPPORTED, "the adjoint needs the Cholesky solver");
     const int64_t rows = sd == 0 ? e->m : e->n, cols = sd == 0 ? e->n : e->m;
     int rc = ALS_OK;
     const uint16_t *Ys = nullptr;
@@ -617,7 +631,7 @@ int als_forward_dev(als_engine *e, int sd, const float *d_Y, float *d_X, const f
     }
     float *Ls = nullptr;
     if (save) {
-        const size_t np = (size_t)e->ld * (e->ld + 1) / 2;
+        const size_t np = (size_t)(e->T * (e->T + 1) / 2) * 256;  // factor tiles per row
         size_t need = (size_t)std::max<int64_t>(1, rows) * np * sizeof(float);
         if (e->Lsave_bytes < need) {
             dev_free(e, e->Lsave);
@@ -634,7 +648,7 @@ int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, c
     if (!e || !d_Y || !d_X || !d_gbar || !d_xbar || !d_lambda_bar) return ALS_E_INVALID;
     if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
     const int64_t rows = sd == 0 ? e->m : e->n, cols = sd == 0 ? e->n : e->m;
-    const size_t np = (size_t)e->ld * (e->ld + 1) / 2;
+    const size_t np = (size_t)(e->T * (e->T + 1) / 2) * 256;
     if (!e->Lsave || e->Lsave_bytes < (size_t)rows * np * sizeof(float))
         return set_err(e, ALS_E_STATE, "run als_forward_dev with save_factor first");
     AdjointArgs a{};

@@ -125,28 +125,32 @@ hipError_t launch_to16(int prec, const float *src, uint16_t *dst, int64_t count,
 //   dL/dlambda_u = -n_u xbar_u . x_u
 template <int T>
 __global__ __launch_bounds__(WAVE) void adjoint_solve_kernel(AdjointArgs a) {
-    constexpr int FP = 16 * T, NP = FP * (FP + 1) / 2;
-    extern __shared__ __attribute__((aligned(16))) float P[];
+    constexpr int FP = 16 * T, NT = T * (T + 1) / 2;
+    __shared__ __attribute__((aligned(16))) float xbuf[16 * MAX_T];
     const int l = lane(), f = a.f;
     for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
         const int n = a.ptr[row + 1] - a.ptr[row];
         const int64_t o = (int64_t)row * FP;
-        float glo = 0.f, ghi = 0.f, lb = 0.f;
+        float g[T];
+#pragma unroll
+        for (int J = 0; J < T; ++J) g[J] = 0.f;
+        float lb = 0.f;
         if (n > 0) {
-            const float *src = a.Lsave + (int64_t)row * NP;
-            for (int e = l; e < NP; e += WAVE) P[e] = src[e];
-            __syncthreads();
-            glo = l < f ? a.Gbar[o + l] : 0.f;
-            ghi = WAVE + l < f ? a.Gbar[o + WAVE + l] : 0.f;
-            solve_l(P, f, glo, ghi);
-            solve_lt(P, f, glo, ghi);
-            const float xlo = l < f ? a.X[o + l] : 0.f;
-            const float xhi = WAVE + l < f ? a.X[o + WAVE + l] : 0.f;
-            lb = -(float)n * wave_sum(glo * xlo + ghi * xhi);
-            __syncthreads();
+            f32x4 L[NT];
+            const float *src = a.Lsave + (int64_t)row * NT * 256;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) L[t] = *reinterpret_cast<const f32x4 *>(src + t * 256 + 4 * l);
+            load_colform<T>(a.Gbar + o, f, g);
+            f32x4 gr[T];
+#pragma unroll
+            for (int I = 0; I < T; ++I) gr[I] = col_to_row(g[I]);
+            fwd_subst<T>(L, f, gr, g);
+            back_subst<T>(L, f, g, xbuf);
+            float x[T];
+            load_colform<T>(a.X + o, f, x);
+            lb = -(float)n * colform_dot(g, x, T);
         }
-        if (l < FP) a.Xbar[o + l] = l < f ? glo : 0.f;
-        if (WAVE + l < FP) a.Xbar[o + WAVE + l] = WAVE + l < f ? ghi : 0.f;
+        store_colform<T>(a.Xbar + o, g);
         if (l == 0) a.lambda_bar[row] = lb;
     }
 }
@@ -179,9 +183,8 @@ __global__ __launch_bounds__(WAVE) void ybar_kernel(YbarArgs a) {
 }
 
 hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s) {
-    const int FP = 16 * T, NP = FP * (FP + 1) / 2;
     const int grid = std::max(1, std::min(a.rows, 65536));
-    const size_t lds = (size_t)NP * 4;
+    const size_t lds = 0;
     switch (T) {
 #define ALS_ADJ(TT) case TT: adjoint_solve_kernel<TT><<<grid, WAVE, lds, s>>>(a); break;
     ALS_ADJ(1) ALS_ADJ(2) ALS_ADJ(3) ALS_ADJ(4) ALS_ADJ(5) ALS_ADJ(6) ALS_ADJ(7) ALS_ADJ(8) ALS_ADJ(9)

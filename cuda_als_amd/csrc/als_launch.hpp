@@ -7,7 +7,7 @@
 namespace als {
 
 enum Prec { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2 };
-enum SolveKind { SOLVE_CHOL = 0, SOLVE_CG = 1 };
+enum SolveKind { SOLVE_CHOL = 0, SOLVE_CG = 1, SOLVE_CHOL_REFINE = 2 };
 constexpr int MAX_T = 9;  // augmented dimension f + 1 <= 16 * MAX_T
 
 struct UpdateArgs {
@@ -33,7 +33,10 @@ struct UpdateArgs {
     int cg_fp16_a;              // CG on an FP16-rounded A
     int refine;                 // FP64-residual refinement steps (Cholesky)
     int *fail_count;            // rows whose solve broke down (x set to 0)
-    float *Lsave;               // packed Cholesky factor per row (nullable)
+    float *Lsave;               // Cholesky factor tiles per row, NT*256 floats (nullable)
+    float *scratch;             // per-wave copy of A for refinement, NT*256 floats per workgroup
+    const float *zero32;        // one row of zeros (16*T floats) read by unused chunk slots
+    const uint16_t *zero16;     // same, 16-bit
 };
 
 struct AdjointArgs {

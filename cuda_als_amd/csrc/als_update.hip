@@ -10,6 +10,22 @@
 namespace als {
 
 template <int S>
+static const void *kernel_w2_for(int T) {
+    switch (T) {
+    case 1: return (const void *)&update_kernel_w2<1, ALS_PREC, S>;
+    case 2: return (const void *)&update_kernel_w2<2, ALS_PREC, S>;
+    case 3: return (const void *)&update_kernel_w2<3, ALS_PREC, S>;
+    case 4: return (const void *)&update_kernel_w2<4, ALS_PREC, S>;
+    case 5: return (const void *)&update_kernel_w2<5, ALS_PREC, S>;
+    case 6: return (const void *)&update_kernel_w2<6, ALS_PREC, S>;
+    case 7: return (const void *)&update_kernel_w2<7, ALS_PREC, S>;
+    case 8: return (const void *)&update_kernel_w2<8, ALS_PREC, S>;
+    case 9: return (const void *)&update_kernel_w2<9, ALS_PREC, S>;
+    default: return nullptr;
+    }
+}
+
+template <int S>
 static const void *kernel_for(int T) {
     switch (T) {
     case 1: return (const void *)&update_kernel<1, ALS_PREC, S>;
@@ -27,14 +43,18 @@ static const void *kernel_for(int T) {
 
 template <int T>
 static int lds_for(int solve, int refine) {
-    return Geo<T, ALS_PREC>::lds_bytes(solve == SOLVE_CHOL && refine > 0);
+    (void)solve;
+    (void)refine;
+    return Geo<T, ALS_PREC>::lds_bytes();
 }
 
 #define ALS_CAT2(a, b) a##b
 #define ALS_CAT(a, b) ALS_CAT2(a, b)
 
 const void *ALS_CAT(update_kernel_ptr_p, ALS_PREC)(int T, int solve) {
-    return solve == SOLVE_CHOL ? kernel_for<SOLVE_CHOL>(T) : kernel_for<SOLVE_CG>(T);
+    return solve == SOLVE_CHOL ? kernel_for<SOLVE_CHOL>(T)
+           : solve == SOLVE_CG ? kernel_for<SOLVE_CG>(T)
+                               : kernel_w2_for<SOLVE_CHOL_REFINE>(T);
 }
 
 int ALS_CAT(update_lds_bytes_p, ALS_PREC)(int T, int solve, int refine) {
