@@ -1,6 +1,6 @@
 """ctypes declarations of the engine's C ABI (include/als.h, include/als_data.h).
 
-The library must have been built (python -m cuda_als_amd.build); there is no
+The library must have been built (python cuda_als_amd/build.py); there is no
 fallback implementation: a missing library raises at import of this module.
 """
 from __future__ import annotations
@@ -34,6 +34,11 @@ class AlsInfo(C.Structure):
                 ("nbatches", C.c_int * 2), ("nunits", C.c_int * 2), ("nsplit_rows", C.c_int * 2),
                 ("split_len", C.c_int64 * 2), ("failures", C.c_int64), ("m", C.c_int64), ("n", C.c_int64),
                 ("nnz", C.c_int64), ("n_test", C.c_int64)]
+
+
+class ScheduleStats(C.Structure):
+    _fields_ = [("nunits", C.c_int64), ("nsplit_rows", C.c_int64), ("nbatches", C.c_int64), ("nslots", C.c_int64),
+                ("split_len", C.c_int64), ("batch_work", C.c_int64)]
 
 
 class SynthSpec(C.Structure):
@@ -74,6 +79,8 @@ SIGNATURES = {
     "als_device_factors": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "als_forward_dev": (C.c_int, [P, C.c_int, P, P, P, C.c_int]),
     "als_adjoint_dev": (C.c_int, [P, C.c_int, P, P, P, P, P, P]),
+    "als_schedule_preview": (C.c_int, [C.c_int64, P, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(ScheduleStats), P, C.c_int64]),
     # als_data.h
     "als_synth_generate": (C.c_int, [C.POINTER(SynthSpec), C.POINTER(P)]),
     "als_dataset_from_csr": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_int64, P, P, P, C.POINTER(P)]),
@@ -93,7 +100,7 @@ SIGNATURES = {
 
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
-        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m cuda_als_amd.build`")
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python cuda_als_amd/build.py`")
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -1,7 +1,8 @@
 """Builds the engine library (gfx950 HIP + host C++) and the oracle.
 
-    python -m cuda_als_amd.build          # everything
-    python -m cuda_als_amd.build --lib    # engine only
+    python cuda_als_amd/build.py          # everything
+    python cuda_als_amd/build.py --lib    # engine only
+(run by path: importing the package loads the library being rebuilt)
 
 Outputs, in the source tree (git-ignored, shipped with the tree):
     cuda_als_amd/libals_mi355x.so   the product: kernels + C ABI (include/als.h,

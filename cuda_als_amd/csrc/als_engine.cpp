@@ -679,3 +679,26 @@ int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, c
 }
 
 }  // extern "C"
+
+extern "C" int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_work, int grid,
+                                    int gram_prec, als_schedule_stats *out, int *units_out, int64_t max_units) {
+    if (!ptr || !out || rows < 0 || grid <= 0 || gram_prec < ALS_PREC_FP32 || gram_prec > ALS_PREC_FP16)
+        return ALS_E_INVALID;
+    for (int64_t r = 0; r < rows; r++)
+        if (ptr[r + 1] < ptr[r]) return ALS_E_INVALID;
+    Schedule sc = build_schedule(rows, ptr, split_len, batch_work, grid, gram_prec);
+    out->nunits = (int64_t)sc.units.size();
+    out->nsplit_rows = (int64_t)sc.splits.size();
+    out->nbatches = (int64_t)sc.batch_begin.size() - 1;
+    out->nslots = sc.nslots;
+    out->split_len = sc.split_len;
+    out->batch_work = sc.batch_work;
+    if (units_out)
+        for (int64_t u = 0; u < (int64_t)sc.units.size() && u < max_units; u++) {
+            units_out[4 * u + 0] = sc.units[u].row;
+            units_out[4 * u + 1] = sc.units[u].kb;
+            units_out[4 * u + 2] = sc.units[u].ke;
+            units_out[4 * u + 3] = sc.units[u].split;
+        }
+    return ALS_OK;
+}

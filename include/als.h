@@ -124,6 +124,18 @@ int als_synchronize(als_engine *e);
 int als_get_info(als_engine *e, als_info *out);
 int als_save_libmf(als_engine *e, const char *path);
 
+/* Host-only (no device needed): the work schedule the engine builds for one
+ * side from its CSR row pointers: whole rows plus parts of rows longer than
+ * split_len, heaviest first, cut into queue batches. units_out (nullable)
+ * receives 4 ints per unit {row, kb, ke, split_id}, at most max_units.
+ * No reference counterpart (the reference launches one block per row in row
+ * order, als_model.cu:1891-1982). */
+typedef struct {
+    int64_t nunits, nsplit_rows, nbatches, nslots, split_len, batch_work;
+} als_schedule_stats;
+int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_work, int grid,
+                         int gram_prec, als_schedule_stats *out, int *units_out, int64_t max_units);
+
 /* Device factor matrices of the engine (row stride als_info.ld floats). */
 int als_device_factors(als_engine *e, int side, float **d_ptr);
 

@@ -42,7 +42,7 @@ _SIG = {
 
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
-        raise ImportError(f"{LIB_PATH} missing: python -m cuda_als_amd.build --oracle")
+        raise ImportError(f"{LIB_PATH} missing: python cuda_als_amd/build.py --oracle")
     lib = C.CDLL(str(LIB_PATH))
     for k, (r, a) in _SIG.items():
         fn = getattr(lib, k)
