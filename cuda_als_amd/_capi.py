@@ -8,6 +8,16 @@ from __future__ import annotations
 import ctypes as C
 from pathlib import Path
 
+# One HIP runtime per process. PyTorch-ROCm ships its own libamdhip64 (SONAME
+# libamdhip64.so.7) and links it as "libamdhip64.so". If torch is loaded first,
+# the engine's NEEDED libamdhip64.so.7 binds to that same copy; the other
+# order loads a second runtime, which then sees no device. So torch, when it
+# is installed, is loaded before the engine.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 LIB_PATH = Path(__file__).resolve().parent / "libals_mi355x.so"
 
 ALS_OK = 0
