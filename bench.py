@@ -108,6 +108,27 @@ def cpu_baseline(d, f, lam, budget_s=20.0):
                        f"{per_row['items'][1]} sampled items, per-row time scaled to all {d.m} users + {d.n} items")}
 
 
+def copy_calibration(torch, nbytes: int, steps: int) -> int:
+    """A plain device-to-device copy of known size: the HBM read/write counters
+    of this kernel give the factor between counter units and bytes."""
+    n = nbytes // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        dst.copy_(src)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(s)
+        for _ in range(steps):
+            dst.copy_(src)
+        t1.record(s)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "ms": ms,
+                      "GB_per_s": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,13 +140,18 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (debug only)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only time a plain device copy of BYTES (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU row sharding is not built "
-              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "here. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
+
+    if a.copy_calibration:
+        return copy_calibration(torch, a.copy_calibration, a.steps)
 
     import cuda_als_amd as ca
     from cuda_als_amd import engine as E
