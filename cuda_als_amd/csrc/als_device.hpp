@@ -85,16 +85,17 @@ template <int T, int E> struct Geo {
     static constexpr int YT_H = 40;             // 16-bit stage row: 32 ratings + 8 pad (80 B)
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
     static constexpr int STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : FP * YT_H * 2;
-    static constexpr int PANEL_BYTES = (T - 1) * 16 * 17 * 4;  // Cholesky panel tiles (after the Gram)
-    static constexpr int MISC_FLOATS = 32 + 32 + 128 + 128;  // ibuf, rbuf, colbuf, xbuf
-    static constexpr int R1_BYTES = STAGE_BYTES > PANEL_BYTES ? STAGE_BYTES : PANEL_BYTES;
-    __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + R1_BYTES; }
+    static constexpr int MISC_FLOATS = 32 + 32;  // ibuf, rbuf
+    __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + STAGE_BYTES; }
 };
 
 // ------------------------------------------------------------------ Gram
-// Accumulates the lower tiles of the augmented Gram of one row into acc.
-// acc[tile_id(I,J)][r] holds G[16I + 4q + r][16J + c] (q = lane/16,
-// c = lane%16), the MFMA 16x16 accumulator layout.
+// Accumulates the lower tiles of the augmented Gram of one row into acc
+// (q = lane/16, c = lane%16): diagonal tiles in the MFMA accumulator layout,
+// acc[tile_id(I,I)][r] = G[16I + 4q + r][16I + c]; off-diagonal tiles as
+// the accumulator of the transposed product, acc[tile_id(I,J)][r] =
+// G[16J + 4q + r][16I + c] = G[16I + c][16J + 4q + r] (I > J), the layout
+// the register Cholesky works in.
 template <int T, int E>
 __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                                          f32x4 (&acc)[Geo<T, E>::NT], char *lds) {
@@ -148,7 +149,7 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
 #pragma unroll
                     for (int J = 0; J <= I; ++J)
                         acc[tile_id(I, J)] =
-                            __builtin_amdgcn_mfma_f32_16x16x4f32(F[I], F[J], acc[tile_id(I, J)], 0, 0, 0);
+                            __builtin_amdgcn_mfma_f32_16x16x4f32(F[J], F[I], acc[tile_id(I, J)], 0, 0, 0);
             }
             __syncthreads();
         }
@@ -207,11 +208,11 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                 for (int J = 0; J <= I; ++J) {
                     if constexpr (E == PREC_BF16)
                         acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, F[I]), __builtin_bit_cast(bf16x8, F[J]), acc[tile_id(I, J)],
+                            __builtin_bit_cast(bf16x8, F[J]), __builtin_bit_cast(bf16x8, F[I]), acc[tile_id(I, J)],
                             0, 0, 0);
                     else
                         acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                            __builtin_bit_cast(f16x8, F[I]), __builtin_bit_cast(f16x8, F[J]), acc[tile_id(I, J)],
+                            __builtin_bit_cast(f16x8, F[J]), __builtin_bit_cast(f16x8, F[I]), acc[tile_id(I, J)],
                             0, 0, 0);
                 }
             __syncthreads();
@@ -238,11 +239,69 @@ __device__ __forceinline__ void add_diag(f32x4 (&acc)[NT], int f, float reg) {
         }
 }
 
+// ------------------------------------------------------- lane exchange
+// A 64-lane wave is 4 rows (q = lane / 16) of 16 lanes (c = lane % 16).
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Lane N of this lane's row (DPP row_newbcast).
+template <int N>
+__device__ __forceinline__ float row_bcast(float v) {
+    return dpp_mov<0x150 + N>(v);
+}
+
+// Sum over the 16 lanes of a row. Quad sums, then mirror and rotate-by-8
+// pairings: every lane adds the same operands in the same order, so all 16
+// lanes hold identical bits.
+__device__ __forceinline__ float row16_sum(float v) {
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x140>(v);  // row_mirror
+    v += dpp_mov<0x128>(v);  // row_ror:8
+    return v;
+}
+__device__ __forceinline__ double row16_sum_d(double v) {
+    v += dpp_mov_d<0xB1>(v);
+    v += dpp_mov_d<0x4E>(v);
+    v += dpp_mov_d<0x140>(v);
+    v += dpp_mov_d<0x128>(v);
+    return v;
+}
+
+// Sum over the 4 rows: permlane16 swap pairs rows (0,1), (2,3); permlane32
+// swap pairs the two halves. Each pair sum is formed from the same two
+// operands in every lane.
+__device__ __forceinline__ float q4_sum(float v) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(s[0]) + __uint_as_float(s[1]);
+}
+__device__ __forceinline__ double q4_sum_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const auto plo = __builtin_amdgcn_permlane16_swap((unsigned int)b, (unsigned int)b, false, false);
+    const auto phi = __builtin_amdgcn_permlane16_swap((unsigned int)(b >> 32), (unsigned int)(b >> 32), false, false);
+    v = __longlong_as_double(((long long)phi[0] << 32) | plo[0]) + __longlong_as_double(((long long)phi[1] << 32) | plo[1]);
+    const long long b2 = __double_as_longlong(v);
+    const auto slo = __builtin_amdgcn_permlane32_swap((unsigned int)b2, (unsigned int)b2, false, false);
+    const auto shi = __builtin_amdgcn_permlane32_swap((unsigned int)(b2 >> 32), (unsigned int)(b2 >> 32), false, false);
+    return __longlong_as_double(((long long)shi[0] << 32) | slo[0]) + __longlong_as_double(((long long)shi[1] << 32) | slo[1]);
+}
+
 // ------------------------------------------------------- vector forms
 // A vector v of the padded dimension 16T lives in one of two register forms:
 //   column form  vc[J]      = v[16J + c]          (replicated over q)
-//   row form     vr[I][r]   = v[16I + 4q + r]     (replicated over c)
-// matching the columns and the rows a lane owns in the accumulator tiles.
+//   row form     vr[J][r]   = v[16J + 4q + r]     (replicated over c)
 
 __device__ __forceinline__ f32x4 col_to_row(float vcJ) {
     const int l = lane(), q = l >> 4;
@@ -251,262 +310,270 @@ __device__ __forceinline__ f32x4 col_to_row(float vcJ) {
     for (int r = 0; r < 4; ++r) o[r] = __shfl(vcJ, (l & 48) | (4 * q + r), WAVE);
     return o;
 }
+__device__ __forceinline__ double col_to_row_d(double vcJ, int r) {
+    const int l = lane(), q = l >> 4;
+    return __shfl(vcJ, (l & 48) | (4 * q + r), WAVE);
+}
 
+// Lane (q, c) takes v[c] from row c/4, where every lane holds the row's four
+// entries; the source lane picks the entry the destination needs.
 __device__ __forceinline__ float row_to_col(const f32x4 &vrJ) {
-    const int c = lane() & 15, src = (c >> 2) << 4;
-    const float t0 = __shfl(vrJ[0], src, WAVE), t1 = __shfl(vrJ[1], src, WAVE);
-    const float t2 = __shfl(vrJ[2], src, WAVE), t3 = __shfl(vrJ[3], src, WAVE);
-    const int r = c & 3;
-    return r == 0 ? t0 : r == 1 ? t1 : r == 2 ? t2 : t3;
+    const int c = lane() & 15;
+    return __shfl(pick4(vrJ, c & 3), ((c >> 2) << 4) | c, WAVE);
 }
 
-// Sum over the 16 lanes of a row (the column index c); every lane gets it.
-__device__ __forceinline__ float row16_sum(float v) {
-#pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
-    return v;
-}
-__device__ __forceinline__ double row16_sum_d(double v) {
-#pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
-    return v;
-}
-// Sum over the 4 row groups (q); every lane gets it.
-__device__ __forceinline__ float q4_sum(float v) {
-    v += __shfl_xor(v, 16, WAVE);
-    return v + __shfl_xor(v, 32, WAVE);
-}
-__device__ __forceinline__ double q4_sum_d(double v) {
-    v += __shfl_xor(v, 16, WAVE);
-    return v + __shfl_xor(v, 32, WAVE);
-}
+// ------------------------------------------------------- tile layout
+// The augmented Gram is held as its lower 16x16 tiles, a[tile_id(I,J)],
+// I >= J, in MFMA accumulator registers (lane (q,c), register r):
+//   diagonal tile   (I,I): A[16I + 4q + r][16I + c]  (full, symmetric)
+//   off-diagonal    (I,J): A[16I + c][16J + 4q + r]  ("row by lane")
+// The Gram produces off-diagonal tiles in this second layout directly (as
+// the transposed upper tile). With it, every product the blocked Cholesky
+// needs takes its MFMA operands straight from registers.
+//
+// After chol_regs, off-diagonal tiles hold L (same layout) and each
+// diagonal tile holds W = L_KK^-1 in the row-by-lane layout:
+// a[tile_id(K,K)][r] at lane (q,c) = W[c][4q + r]. Row f of the last block
+// row (the augmented row) holds z = L^-1 b for blocks K < T-1.
 
-// Row f of the augmented tiles (b before the factorisation, z = L^-1 b
-// after it), as a column-form vector masked to the first f entries. Row f
-// lies in the last block row (f >> 4 == T - 1 for T = ceil((f+1)/16)).
+// Row f of the augmented matrix before the factorisation (b, masked to
+// the first f entries), column form.
 template <int T, int NT>
-__device__ __forceinline__ void extract_row_f(const f32x4 (&a)[NT], int f, float (&vc)[T]) {
+__device__ __forceinline__ void extract_b(const f32x4 (&a)[NT], int f, float (&bc)[T]) {
     const int l = lane(), c = l & 15;
-    const int src = (((f & 15) >> 2) << 4) | c, rf = f & 3;
 #pragma unroll
-    for (int J = 0; J < T; ++J) {
-        const float v = __shfl(pick4(a[tile_id(T - 1, J)], rf), src, WAVE);
-        vc[J] = (16 * J + c < f) ? v : 0.f;
+    for (int K = 0; K < T; ++K) {
+        // lane (q, f & 15) holds A[f][16K + 4q + r] (off-diagonal layout, or
+        // column f of the symmetric diagonal tile)
+        f32x4 br;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) br[r] = __shfl(a[tile_id(T - 1, K)][r], (l & 48) | (f & 15), WAVE);
+        const float v = row_to_col(br);
+        bc[K] = (16 * K + c < f) ? v : 0.f;
     }
 }
 
 // ------------------------------------------------------------- Cholesky
-// Blocked right-looking Cholesky of the leading f x f block of the
-// augmented matrix, in place in the accumulator tiles (16-column panels).
-//  * Panel: for each pivot column k the raw column is published through
-//    LDS (colbuf); the panel's later columns take the rank-1 update
-//    A[i][j] -= A[i][k] A[j][k] / A[k][k]. Columns stay unscaled until the
-//    panel is done, then every lane scales its own column by 1/sqrt(pivot).
-//  * Trailing tiles: A_IJ -= L_IK L_JK^T for the whole panel at once on FP32
-//    MFMA (16x16x4), the panel tiles passing through LDS to get the MFMA
-//    operand layout.
-// Row f of the result is z = L^-1 b. ok becomes false if a pivot was <= 0.
-template <int T, int K, int R, int NT>
-__device__ __forceinline__ void chol_step(f32x4 (&a)[NT], int cc4, int kend, float *colbuf, bool &ok,
-                                          float &myinv) {
-    const int cc = cc4 + R;  // cc & 3 == R: static register selection below
-    if (cc >= kend) return;
-    const int l = lane(), q = l >> 4, c = l & 15;
-    const float d = bcast_lane(a[tile_id(K, K)][R], (cc4 << 2) | cc);
-    const bool good = d > 0.f;
-    ok = ok && good;
-    const float rs = good ? __builtin_amdgcn_rsqf(d) : 0.f;
-    if (c == cc) {
-        myinv = rs;
+// Step J of the diagonal-block factorisation: pivot J of D (standard
+// layout, kept symmetric), its column scaled by 1/sqrt(pivot), the rank-1
+// update of the trailing part, and the matching step of W = L^-1 built by
+// forward elimination (W[J] /= L[J][J]; W[c] -= L[c][J] W[J] for c > J),
+// with W held row by lane.
+template <int J>
+__device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, int jend, bool &ok) {
+    if constexpr (J < 16) {
+        if (J < jend) {
+            const int l = lane(), q = l >> 4, c = l & 15;
+            constexpr int JG = J >> 2, JR = J & 3;
+            const float piv = bcast_lane(d[JR], (JG << 4) | J);
+            const bool good = piv > 0.f;
+            ok = ok && good;
+            const float rs = good ? __builtin_amdgcn_rsqf(piv) : 0.f;
+            const float lc = __shfl(d[JR], (JG << 4) | c, WAVE) * rs;  // L[c][J] = D[J][c] / sqrt(piv)
 #pragma unroll
-        for (int I = K; I < T; ++I) {
-            f32x4 v = a[tile_id(I, K)];
-            if (I == K) {
+            for (int r = 0; r < 4; ++r) {
+                const float li = row_bcast<J>(d[r]) * rs;  // L[4q+r][J]
+                const float wj = row_bcast<J>(w[r]) * rs;  // W[J][4q+r] / L[J][J]
+                d[r] = (4 * q + r > J && c > J) ? d[r] - li * lc : d[r];
+                w[r] = (c == J) ? wj : ((c > J) ? w[r] - lc * wj : w[r]);
+            }
+        }
+        diag_steps<J + 1>(d, w, jend, ok);
+    }
+}
+
+// Factor the diagonal block with jend pivots; on return the tile holds W
+// row by lane, rows >= jend zero.
+__device__ __forceinline__ void diag_factor(f32x4 &d, int jend, bool &ok) {
+    const int l = lane(), q = l >> 4, c = l & 15;
+    f32x4 w;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[r] = (c == 4 * q + r) ? 1.f : 0.f;
+    diag_steps<0>(d, w, jend, ok);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d[r] = (c < jend) ? w[r] : 0.f;
+}
+
+// Blocked right-looking Cholesky, panel K:
+//   diagonal block -> W_KK = L_KK^-1 (diag_factor)
+//   panel          L_IK^T = W_KK A_IK^T       (4 MFMAs per tile)
+//   trailing       A_IJ  -= L_IK L_JK^T        (4 MFMAs per tile)
+// The last block holds row f (the augmented row, never a pivot): yl gets
+// its reduced entries (row form) just before that block is factored.
+template <int T, int K, int NT>
+__device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, bool &ok, f32x4 &yl) {
+    if constexpr (K < T) {
+        const int jend = min(16, f - 16 * K);
+        if constexpr (K == T - 1) {
+            const int l = lane();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) yl[r] = __shfl(a[tile_id(K, K)][r], (l & 48) | (f & 15), WAVE);
+        }
+        diag_factor(a[tile_id(K, K)], jend, ok);
+        if constexpr (K + 1 < T) {
+#pragma unroll
+            for (int I = K + 1; I < T; ++I) {
+                const f32x4 o = a[tile_id(I, K)];
+                f32x4 nw = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    if (4 * q + r < cc) v[r] = 0.f;
-            }
-            *reinterpret_cast<f32x4 *>(colbuf + 16 * I + 4 * q) = v;
-        }
-    }
-    __syncthreads();
-    const float lk = (c > cc) ? colbuf[16 * K + c] * (rs * rs) : 0.f;
-#pragma unroll
-    for (int I = K; I < T; ++I) a[tile_id(I, K)] -= *reinterpret_cast<const f32x4 *>(colbuf + 16 * I + 4 * q) * lk;
-    __syncthreads();
-}
-
-template <int T, int K, int NT>
-__device__ __forceinline__ void chol_panel(f32x4 (&a)[NT], int f, float *colbuf, float *panel, bool &ok) {
-    if constexpr (K < T) {
-        if (16 * K < f) {
-            const int l = lane(), q = l >> 4, c = l & 15;
-            const int kend = min(16, f - 16 * K);
-            float myinv = 1.f;
-            for (int cc4 = 0; cc4 < kend; cc4 += 4) {
-                chol_step<T, K, 0>(a, cc4, kend, colbuf, ok, myinv);
-                chol_step<T, K, 1>(a, cc4, kend, colbuf, ok, myinv);
-                chol_step<T, K, 2>(a, cc4, kend, colbuf, ok, myinv);
-                chol_step<T, K, 3>(a, cc4, kend, colbuf, ok, myinv);
+                    nw = __builtin_amdgcn_mfma_f32_16x16x4f32(a[tile_id(K, K)][r], o[r], nw, 0, 0, 0);
+                a[tile_id(I, K)] = nw;
             }
 #pragma unroll
-            for (int I = K; I < T; ++I) a[tile_id(I, K)] *= myinv;
-            if constexpr (K + 1 < T) {
-                constexpr int TS = 16 * 17;  // one panel tile, rows padded to 17
+            for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int I = K + 1; I < T; ++I)
+                for (int J = K + 1; J < T; ++J) {
+                    const float nf = -a[tile_id(J, K)][r];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) panel[(I - K - 1) * TS + (4 * q + r) * 17 + c] = a[tile_id(I, K)][r];
-                __syncthreads();
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    float F[T];
-#pragma unroll
-                    for (int I = K + 1; I < T; ++I) F[I] = panel[(I - K - 1) * TS + (l & 15) * 17 + 4 * s + (l >> 4)];
-#pragma unroll
-                    for (int I = K + 1; I < T; ++I) {
-                        const float nf = -F[I];
-#pragma unroll
-                        for (int J = K + 1; J <= I; ++J)
-                            a[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x4f32(nf, F[J], a[tile_id(I, J)], 0, 0, 0);
-                    }
+                    for (int I = J; I < T; ++I)
+                        a[tile_id(I, J)] =
+                            __builtin_amdgcn_mfma_f32_16x16x4f32(nf, a[tile_id(I, K)][r], a[tile_id(I, J)], 0, 0, 0);
                 }
-                __syncthreads();
-            }
         }
-        chol_panel<T, K + 1>(a, f, colbuf, panel, ok);
+        chol_panels<T, K + 1>(a, f, ok, yl);
     }
 }
 
+// Factorises the augmented matrix in place and returns z = L^-1 b in row
+// form (zero past f). ok becomes false if a pivot was not positive.
 template <int T, int NT>
-__device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, float *colbuf, float *panel) {
+__device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, f32x4 (&zr)[T]) {
     bool ok = true;
-    chol_panel<T, 0>(a, f, colbuf, panel, ok);
+    f32x4 yl;
+    chol_panels<T, 0>(a, f, ok, yl);
+    const int l = lane(), q = l >> 4, c = l & 15;
+#pragma unroll
+    for (int K = 0; K + 1 < T; ++K)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zr[K][r] = __shfl(a[tile_id(T - 1, K)][r], (l & 48) | (f & 15), WAVE);
+    // last block: z = W y (W rows past the pivots are zero)
+    const int jend = f - 16 * (T - 1);
+    float u = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) u += a[tile_id(T - 1, T - 1)][r] * ((4 * q + r < jend) ? yl[r] : 0.f);
+    const float zc = q4_sum(u);
+    zr[T - 1] = col_to_row(c < jend ? zc : 0.f);
     return ok;
 }
 
 // -------------------------------------------------- triangular solves
-// On the factor held in the tiles (L[16I+4q+r][16J+c] in a[tile(I,J)][r]),
-// block by block: the diagonal 16x16 block is solved column by column with
-// cross-lane broadcasts, the off-diagonal blocks enter as tile mat-vecs.
-
-// Diagonal block of L^T x = y (column form in/out): column j, j = jend-1..0.
-template <int R>
-__device__ __forceinline__ void bsub_step(const f32x4 &D, int j4, int jend, float &y, float &xk) {
-    const int j = j4 + R;
-    if (j >= jend) return;
-    const int c = lane() & 15;
-    const float yj = bcast_lane(y, j);
-    const float ljj = bcast_lane(D[R], (j4 << 2) | j);
-    const float xj = yj * __builtin_amdgcn_rcpf(ljj);
-    xk = (c == j) ? xj : xk;
-    const float lji = __shfl(D[R], (j4 << 2) | c, WAVE);  // L[j][c] of this block
-    y = (c < j) ? y - lji * xj : y;
-}
-
-// L^T x = z. v: column form, z in, x out (entries >= f are zero). The
-// solved blocks also go to the wave's LDS vector buffer xbuf (16T floats),
-// where later blocks read them in row form with one 16-byte load; on return
-// xbuf holds x.
+// L^T x = z. z in row form; x out in column form (zero past f).
+//   x_K = W_KK^T (z_K - sum_{I>K} L_IK^T x_I)
 template <int T, int NT>
-__device__ __forceinline__ void back_subst(const f32x4 (&a)[NT], int f, float (&v)[T], float *xbuf) {
-    const int l = lane(), q = l >> 4, c = l & 15;
-    const int KP = (f - 1) >> 4;
+__device__ __forceinline__ void back_solve(const f32x4 (&a)[NT], int f, const f32x4 (&zr)[T], float (&xc)[T]) {
+    const int c = lane() & 15;
 #pragma unroll
     for (int K = T - 1; K >= 0; --K) {
-        if (K > KP) {
-            v[K] = 0.f;
-            if (q == 0) xbuf[16 * K + c] = 0.f;
-            __syncthreads();
-            continue;
+        f32x4 p = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int I = K + 1; I < T; ++I) p += a[tile_id(I, K)] * xc[I];
+        f32x4 t;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) t[r] = (K + 1 < T) ? zr[K][r] - row16_sum(p[r]) : zr[K][r];
+        const float tc = row_to_col(t);
+        f32x4 xr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xr[r] = row16_sum(a[tile_id(K, K)][r] * tc);
+        const float v = row_to_col(xr);
+        xc[K] = (16 * K + c < f) ? v : 0.f;
+    }
+}
+
+// L w = v. v in column form (entries past f are ignored); w out in row form
+// (zero past f).
+//   w_K = W_KK (v_K - sum_{J<K} L_KJ w_J)
+template <int T, int NT>
+__device__ __forceinline__ void fwd_solve(const f32x4 (&a)[NT], int f, const float (&vc)[T], f32x4 (&wr)[T]) {
+    const int q = lane() >> 4;
+#pragma unroll
+    for (int K = 0; K < T; ++K) {
+        float s = 0.f;
+#pragma unroll
+        for (int J = 0; J < K; ++J)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s += a[tile_id(K, J)][r] * wr[J][r];
+        const float tc = (K > 0) ? vc[K] - q4_sum(s) : vc[K];
+        const f32x4 tr = col_to_row(tc);
+        float u = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) u += a[tile_id(K, K)][r] * tr[r];
+        const f32x4 w = col_to_row(q4_sum(u));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) wr[K][r] = (16 * K + 4 * q + r < f) ? w[r] : 0.f;
+    }
+}
+
+// ------------------------------------------------------ refinement
+// One step of iterative refinement, x += A^-1 (b - A x): the residual is
+// formed in FP64 from Ag, this wave's scratch copy of the augmented tiles
+// taken before the factorisation (row f holds b); the correction is solved
+// on the FP32 factor in the registers.
+template <int T, int NT>
+__device__ __forceinline__ void refine_step(const f32x4 (&a)[NT], int f, const float *Ag, float (&xc)[T]) {
+    const int l = lane(), q = l >> 4, c = l & 15;
+    f32x4 xr[T];
+#pragma unroll
+    for (int K = 0; K < T; ++K) xr[K] = col_to_row(xc[K]);
+    float rc[T];
+#pragma unroll
+    for (int K = 0; K < T; ++K) {
+        // Keep this block's scratch loads from being hoisted into earlier
+        // blocks (that would hold every tile of A in registers at once).
+        __builtin_amdgcn_sched_barrier(0);
+        // (A x)_K in column form: sum_{J<=K} A[16K+c][16J+4q+r] x[16J+4q+r]
+        // (the diagonal tile is symmetric) ...
+        double cp = 0.0;
+#pragma unroll
+        for (int J = 0; J <= K; ++J) {
+            const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(K, J) * 256 + 4 * l);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cp += (double)t[r] * (double)xr[J][r];
         }
-        float p = 0.f;
+        // ... plus sum_{I>K} A[16I+c'][16K+4q+r] x[16I+c'] in row form.
+        double rp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int I = K + 1; I < T; ++I) {
-            const f32x4 xr = *reinterpret_cast<const f32x4 *>(xbuf + 16 * I + 4 * q);
+            const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(I, K) * 256 + 4 * l);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) p += a[tile_id(I, K)][r] * xr[r];
+            for (int r = 0; r < 4; ++r) rp[r] += (double)t[r] * (double)xc[I];
         }
-        float y = v[K] - q4_sum(p);
-        const int jend = min(16, f - 16 * K);
-        float xk = 0.f;
-        for (int j4 = (jend - 1) & ~3; j4 >= 0; j4 -= 4) {
-            bsub_step<3>(a[tile_id(K, K)], j4, jend, y, xk);
-            bsub_step<2>(a[tile_id(K, K)], j4, jend, y, xk);
-            bsub_step<1>(a[tile_id(K, K)], j4, jend, y, xk);
-            bsub_step<0>(a[tile_id(K, K)], j4, jend, y, xk);
+        const double cs = q4_sum_d(cp);
+        // b_K in row form: lane (q, f & 15) of tile (T-1, K), register r.
+        const f32x4 br = *reinterpret_cast<const f32x4 *>(Ag + tile_id(T - 1, K) * 256 + 4 * ((q << 4) | (f & 15)));
+        f32x4 res;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double y = col_to_row_d(cs, r) + row16_sum_d(rp[r]);
+            res[r] = (16 * K + 4 * q + r < f) ? (float)((double)br[r] - y) : 0.f;
         }
-        v[K] = xk;
-        if (q == 0) xbuf[16 * K + c] = xk;
-        __syncthreads();
+        rc[K] = row_to_col(res);
     }
-}
-
-// Diagonal block of L w = y (row form y, w column form): column j = 0..jend-1.
-template <int R>
-__device__ __forceinline__ void fsub_step(const f32x4 &D, int j4, int jend, f32x4 &y, float &wk) {
-    const int j = j4 + R;
-    if (j >= jend) return;
-    const int l = lane(), q = l >> 4, c = l & 15;
-    const float yj = bcast_lane(y[R], j4 << 2);
-    const float ljj = bcast_lane(D[R], (j4 << 2) | j);
-    const float wj = yj * __builtin_amdgcn_rcpf(ljj);
-    wk = (c == j) ? wj : wk;
+    f32x4 wr[T];
+    fwd_solve<T>(a, f, rc, wr);
+    float dc[T];
+    back_solve<T>(a, f, wr, dc);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float lij = __shfl(D[k], (l & 48) | j, WAVE);  // L[4q+k][j] of this block
-        y[k] = (4 * q + k > j) ? y[k] - lij * wj : y[k];
-    }
+    for (int K = 0; K < T; ++K) xc[K] += dc[K];
+    (void)c;
 }
 
-// Forward substitution for block K given the block's right-hand side y (row
-// form, already reduced by the earlier blocks). Returns w_K (column form).
-template <int T, int K, int NT>
-__device__ __forceinline__ float fsub_block(const f32x4 (&a)[NT], int f, f32x4 y) {
-    const int jend = min(16, f - 16 * K);
-    float wk = 0.f;
-    for (int j4 = 0; j4 < jend; j4 += 4) {
-        fsub_step<0>(a[tile_id(K, K)], j4, jend, y, wk);
-        fsub_step<1>(a[tile_id(K, K)], j4, jend, y, wk);
-        fsub_step<2>(a[tile_id(K, K)], j4, jend, y, wk);
-        fsub_step<3>(a[tile_id(K, K)], j4, jend, y, wk);
-    }
-    return wk;
+__device__ __forceinline__ float colform_dot(const float *a, const float *b, int T) {
+    float s = 0.f;
+    for (int J = 0; J < T; ++J) s += a[J] * b[J];
+    return row16_sum(s);
 }
 
-template <int T, int K, int NT>
-__device__ __forceinline__ void fwd_blocks(const f32x4 (&a)[NT], int f, const f32x4 (&vr)[T], float (&w)[T]) {
-    if constexpr (K < T) {
-        w[K] = 0.f;
-        if (16 * K < f) {
-            f32x4 u = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int J = 0; J < K; ++J) u += a[tile_id(K, J)] * w[J];
-            f32x4 y = vr[K];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) y[r] -= row16_sum(u[r]);
-            w[K] = fsub_block<T, K>(a, f, y);
-        }
-        fwd_blocks<T, K + 1>(a, f, vr, w);
-    }
-}
-
-// L w = r. vr: row form in (entries >= f ignored); w: column form out.
-template <int T, int NT>
-__device__ __forceinline__ void fwd_subst(const f32x4 (&a)[NT], int f, const f32x4 (&vr)[T], float (&w)[T]) {
-    fwd_blocks<T, 0>(a, f, vr, w);
-}
-
-// ------------------------------------------------------------ mat-vecs
-// y = A v for the symmetric matrix held as lower tiles (full diagonal
-// tiles) in the accumulator layout, restricted to the leading f x f block.
-// v in column form (zero past f); y returned in column form (zero past f).
-// Lower tile (I,J), lane (q,c): A[16I+4q+r][16J+c] contributes
-//   to y[16I+4q+r] with v[16J+c]   (row partial, summed over c)
-//   to y[16J+c]   with v[16I+4q+r] (column partial, summed over q; J < I)
+// ------------------------------------------------------------ mat-vec
+// y = A v for the symmetric matrix held as tiles (layout above, before any
+// factorisation), restricted to the leading f x f block. v and y in column
+// form (zero past f).
+//   diagonal (I,I): y[16I+4q+r] += A v[16I+c]        row-form partial
+//   off (I,J):      y[16I+c]    += A v[16J+4q+r]     column-form partial
+//                   y[16J+4q+r] += A v[16I+c]        row-form partial
 template <int T, int NT>
 __device__ __forceinline__ void matvec_f32(const f32x4 (&A)[NT], int f, const float (&v)[T], float (&y)[T]) {
-    const int c = lane() & 15, q = lane() >> 4;
+    const int c = lane() & 15;
     f32x4 vr[T];
 #pragma unroll
     for (int I = 0; I < T; ++I) vr[I] = col_to_row(v[I]);
@@ -522,130 +589,21 @@ __device__ __forceinline__ void matvec_f32(const f32x4 (&A)[NT], int f, const fl
 #pragma unroll
         for (int J = 0; J <= I; ++J) {
             const f32x4 t = A[tile_id(I, J)];
-            pr[I] += t * v[J];
-            if (J < I) pc[J] += t[0] * vr[I][0] + t[1] * vr[I][1] + t[2] * vr[I][2] + t[3] * vr[I][3];
+            if (J == I) {
+                pr[I] += t * v[I];
+            } else {
+                pc[I] += t[0] * vr[J][0] + t[1] * vr[J][1] + t[2] * vr[J][2] + t[3] * vr[J][3];
+                pr[J] += t * v[I];
+            }
         }
 #pragma unroll
     for (int I = 0; I < T; ++I) {
+        f32x4 s;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) pr[I][r] = row16_sum(pr[I][r]);
-        const float yc = row_to_col(pr[I]) + q4_sum(pc[I]);
+        for (int r = 0; r < 4; ++r) s[r] = row16_sum(pr[I][r]);
+        const float yc = row_to_col(s) + q4_sum(pc[I]);
         y[I] = (16 * I + c < f) ? yc : 0.f;
     }
-    (void)q;
-}
-
-// r = b - A x with A read back from this wave's global scratch copy of the
-// tiles; products and sums in FP64. x, b in column form; r returned in row
-// form (FP32, zero past f) for the forward substitution.
-template <int T>
-__device__ __forceinline__ void residual_fp64(const float *Ag, int f, const float (&b)[T], const float (&x)[T],
-                                              f32x4 (&r)[T]) {
-    const int l = lane(), q = l >> 4;
-    f32x4 xr[T];
-#pragma unroll
-    for (int I = 0; I < T; ++I) xr[I] = col_to_row(x[I]);
-    // One output block J at a time: y_J = sum_{I>=J} A_IJ^T x_I (column
-    // partials; the diagonal tile is full and symmetric) + sum_{I<J} A_JI x_I
-    // (row partials). Tiles are re-read from the scratch copy (L2-resident).
-#pragma unroll
-    for (int J = 0; J < T; ++J) {
-        double pc = 0.0, pr[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int I = J; I < T; ++I) {
-            const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(I, J) * 256 + 4 * l);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) pc += (double)t[k] * (double)xr[I][k];
-        }
-#pragma unroll
-        for (int I = 0; I < J; ++I) {
-            const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(J, I) * 256 + 4 * l);
-            const double xi = (double)x[I];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) pr[k] += (double)t[k] * xi;
-        }
-        const double pcs = q4_sum_d(pc);        // column form: y_J[c]
-        const f32x4 br = col_to_row(b[J]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double colpart = __shfl(pcs, (l & 48) | (4 * q + k), WAVE);  // -> row form
-            const int i = 16 * J + 4 * q + k;
-            const double y = row16_sum_d(pr[k]) + colpart;
-            r[J][k] = i < f ? (float)((double)br[k] - y) : 0.f;
-        }
-    }
-}
-
-// One step of iterative refinement, x += A^-1 (b - A x), with the residual
-// in FP64 and the correction solved on the FP32 factor. The residual of
-// block K is formed just before block K of the forward substitution needs
-// it, so only one block of it is ever live. Ag is this wave's scratch copy
-// of the augmented tiles before the factorisation (row f holds b).
-template <int T, int K, int NT>
-__device__ __forceinline__ void refine_blocks(const f32x4 (&a)[NT], int f, const float *Ag, const float (&x)[T],
-                                              const float *xbuf, float (&w)[T]) {
-    if constexpr (K < T) {
-        w[K] = 0.f;
-        // Keep the scratch-tile loads of this block from being hoisted into
-        // earlier blocks (that would hold every tile of A in registers).
-        __builtin_amdgcn_sched_barrier(0);
-        if (16 * K < f) {
-            const int l = lane(), q = l >> 4;
-            const int bsrc = ((f & 15) >> 2) << 4, brf = f & 3;
-            double pc = 0.0, pr[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int I = K; I < T; ++I) {
-                const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(I, K) * 256 + 4 * l);
-                const f32x4 xr = *reinterpret_cast<const f32x4 *>(xbuf + 16 * I + 4 * q);
-#pragma unroll
-                for (int k = 0; k < 4; ++k) pc += (double)t[k] * (double)xr[k];
-            }
-#pragma unroll
-            for (int I = 0; I < K; ++I) {
-                const f32x4 t = *reinterpret_cast<const f32x4 *>(Ag + tile_id(K, I) * 256 + 4 * l);
-                const double xi = (double)x[I];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) pr[k] += (double)t[k] * xi;
-            }
-            const double pcs = q4_sum_d(pc);
-            const float *brow = Ag + tile_id(T - 1, K) * 256;  // row f lies in block T-1
-            f32x4 u = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int J = 0; J < K; ++J) u += a[tile_id(K, J)] * w[J];
-            f32x4 y;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int i = 16 * K + 4 * q + k;
-                const double colpart = __shfl(pcs, (l & 48) | (4 * q + k), WAVE);
-                const double bk = (double)brow[4 * (bsrc | (4 * q + k)) + brf];
-                const double rs = row16_sum_d(pr[k]);
-                y[k] = (i < f ? (float)(bk - rs - colpart) : 0.f) - row16_sum(u[k]);
-            }
-            w[K] = fsub_block<T, K>(a, f, y);
-        }
-        refine_blocks<T, K + 1>(a, f, Ag, x, xbuf, w);
-    }
-}
-
-template <int T, int NT>
-__device__ __forceinline__ void refine_step(const f32x4 (&a)[NT], int f, const float *Ag, float (&x)[T],
-                                            float *xbuf) {
-    float w[T];
-    refine_blocks<T, 0>(a, f, Ag, x, xbuf, w);
-    back_subst<T>(a, f, w, xbuf);  // xbuf now holds the correction
-    const int q = lane() >> 4, c = lane() & 15;
-#pragma unroll
-    for (int J = 0; J < T; ++J) {
-        x[J] += w[J];
-        if (q == 0) xbuf[16 * J + c] = x[J];
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ float colform_dot(const float *a, const float *b, int T) {
-    float s = 0.f;
-    for (int J = 0; J < T; ++J) s += a[J] * b[J];
-    return row16_sum(s);
 }
 
 // cuMF CG (cumf_als_cg/cg.cu:37-232) on the register tiles: warm start x,
@@ -720,9 +678,6 @@ template <int T, int E, int S>
 __device__ __forceinline__ void update_body(const UpdateArgs &a) {
     using G = Geo<T, E>;
     extern __shared__ __attribute__((aligned(16))) char lds[];
-    float *colbuf = reinterpret_cast<float *>(lds) + 64;
-    float *xbuf = colbuf + 128;
-    float *stage = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);  // reused as the panel buffer
     const int l = lane();
     const int f = a.f;
     float *Ascr = a.scratch ? a.scratch + (int64_t)blockIdx.x * G::NT * 256 : nullptr;
@@ -775,11 +730,11 @@ __device__ __forceinline__ void update_body(const UpdateArgs &a) {
 #pragma unroll
                         for (int t = 0; t < G::NT; ++t) *reinterpret_cast<f32x4 *>(Ascr + t * 256 + 4 * l) = acc[t];
                     }
-                    ok = chol_regs<T>(acc, f, colbuf, stage);
-                    extract_row_f<T>(acc, f, x);
-                    back_subst<T>(acc, f, x, xbuf);
+                    f32x4 zr[T];
+                    ok = chol_regs<T>(acc, f, zr);
+                    back_solve<T>(acc, f, zr, x);
                     if constexpr (S == SOLVE_CHOL_REFINE) {
-                        for (int s = 0; s < a.refine; ++s) refine_step<T>(acc, f, Ascr, x, xbuf);
+                        for (int s = 0; s < a.refine; ++s) refine_step<T>(acc, f, Ascr, x);
                     }
                     if (a.Lsave) {
                         float *dst = a.Lsave + (int64_t)row * G::NT * 256;
@@ -788,7 +743,7 @@ __device__ __forceinline__ void update_body(const UpdateArgs &a) {
                     }
                 } else {
                     float bvec[T];
-                    extract_row_f<T>(acc, f, bvec);
+                    extract_b<T>(acc, f, bvec);
                     if (a.cg_fp16_a) {
 #pragma unroll
                         for (int t = 0; t < G::NT; ++t)

@@ -126,7 +126,6 @@ hipError_t launch_to16(int prec, const float *src, uint16_t *dst, int64_t count,
 template <int T>
 __global__ __launch_bounds__(WAVE) void adjoint_solve_kernel(AdjointArgs a) {
     constexpr int FP = 16 * T, NT = T * (T + 1) / 2;
-    __shared__ __attribute__((aligned(16))) float xbuf[16 * MAX_T];
     const int l = lane(), f = a.f;
     for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
         const int n = a.ptr[row + 1] - a.ptr[row];
@@ -141,11 +140,9 @@ __global__ __launch_bounds__(WAVE) void adjoint_solve_kernel(AdjointArgs a) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) L[t] = *reinterpret_cast<const f32x4 *>(src + t * 256 + 4 * l);
             load_colform<T>(a.Gbar + o, f, g);
-            f32x4 gr[T];
-#pragma unroll
-            for (int I = 0; I < T; ++I) gr[I] = col_to_row(g[I]);
-            fwd_subst<T>(L, f, gr, g);
-            back_subst<T>(L, f, g, xbuf);
+            f32x4 wr[T];
+            fwd_solve<T>(L, f, g, wr);
+            back_solve<T>(L, f, wr, g);
             float x[T];
             load_colform<T>(a.X + o, f, x);
             lb = -(float)n * colform_dot(g, x, T);
