@@ -85,11 +85,106 @@ template <int T, int E> struct Geo {
     static constexpr int YT_H = 40;             // 16-bit stage row: 32 ratings + 8 pad (80 B)
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
     static constexpr int STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : FP * YT_H * 2;
-    static constexpr int MISC_FLOATS = 32 + 32;  // ibuf, rbuf
+    // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
+    static constexpr int MISC_FLOATS = 32 + 32 + FP + 64;
     __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + STAGE_BYTES; }
 };
 
 // ------------------------------------------------------------------ Gram
+// Zero the stage columns no chunk writes (padding factors past the loaded
+// segments), once per row.
+template <int T, int E>
+__device__ __forceinline__ void stage_clear(int f, char *stage) {
+    using G = Geo<T, E>;
+    const int l = lane();
+    if constexpr (E == PREC_FP32) {
+        float *Ys = reinterpret_cast<float *>(stage);
+        const int nseg = (f + 3) >> 2;  // float4 segments carrying data
+        for (int e = l; e < CHUNK * (G::FP - 4 * nseg); e += WAVE) {
+            int k = e / (G::FP - 4 * nseg), col = 4 * nseg + e % (G::FP - 4 * nseg);
+            Ys[k * G::YR + col] = 0.f;
+        }
+    } else {
+        uint16_t *Yt = reinterpret_cast<uint16_t *>(stage);
+        const int nseg = (f + 7) >> 3;  // 16-byte segments carrying data
+        for (int e = l; e < (G::FP - 8 * nseg) * CHUNK; e += WAVE) {
+            int j = 8 * nseg + e / CHUNK, k = e % CHUNK;
+            Yt[j * G::YT_H + k] = 0;
+        }
+    }
+}
+
+// Stage ratings k0 .. k0+nv-1 of the row starting at rs (nv <= 32): their
+// item indices and values to ibuf/rbuf, the gathered rows y_i of the fixed
+// factors to the stage (FP32: Ys[rating][factor]; 16-bit: Yt[factor][rating]),
+// with the rating in column f. Slots past nv hold zero rows. Ends with the
+// stage visible to the whole wave.
+template <int T, int E>
+__device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int k0, int nv, char *lds) {
+    using G = Geo<T, E>;
+    const int l = lane();
+    const int f = a.f;
+    int *ibuf = reinterpret_cast<int *>(lds);
+    float *rbuf = reinterpret_cast<float *>(lds) + 32;
+    char *stage = lds + G::MISC_FLOATS * 4;
+    if (l < CHUNK) {
+        ibuf[l] = l < nv ? a.idx[rs + k0 + l] : -1;
+        rbuf[l] = l < nv ? a.val[rs + k0 + l] : 0.f;
+    }
+    __syncthreads();
+    if constexpr (E == PREC_FP32) {
+        float *Ys = reinterpret_cast<float *>(stage);
+        const int nseg = (f + 3) >> 2;
+        const int dk = WAVE / nseg, ds = WAVE % nseg;
+        int k = l / nseg, s = l % nseg;
+        for (int task = l; task < CHUNK * nseg; task += WAVE) {
+            const int id = ibuf[k];
+            const float *src = id >= 0 ? a.Y + (int64_t)id * G::FP + 4 * s : a.zero32 + 4 * s;
+            const f32x4 v = *reinterpret_cast<const f32x4 *>(src);
+            *reinterpret_cast<f32x4 *>(Ys + k * G::YR + 4 * s) = v;
+            k += dk;
+            s += ds;
+            if (s >= nseg) { s -= nseg; ++k; }
+        }
+        __syncthreads();
+        if (l < CHUNK) Ys[l * G::YR + f] = rbuf[l];
+    } else {
+        uint16_t *Yt = reinterpret_cast<uint16_t *>(stage);
+        const uint16_t *Ysh = a.Ys;
+        const int nseg = (f + 7) >> 3;
+        // Task = (group of 8 ratings g, segment s of 8 factors): eight
+        // 16-byte loads, an 8x8 transpose of 16-bit values in registers,
+        // eight 16-byte LDS writes of Yt[8s+e][8g..8g+7].
+        for (int task = l; task < 4 * nseg; task += WAVE) {
+            const int g = task / nseg, s = task - g * nseg;
+            u32x4 d[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                // Unused slots of the last chunk read a row of zeros: the
+                // load itself is unconditional.
+                const int id = ibuf[8 * g + e];
+                const uint16_t *src = id >= 0 ? Ysh + (int64_t)id * G::FP + 8 * s : a.zero16 + 8 * s;
+                d[e] = *reinterpret_cast<const u32x4 *>(src);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                u32x4 lo, hi;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const uint32_t x0 = d[2 * w][v], x1 = d[2 * w + 1][v];
+                    lo[w] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);  // low halves of x0, x1
+                    hi[w] = __builtin_amdgcn_perm(x1, x0, 0x07060302u);  // high halves
+                }
+                *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v) * G::YT_H + 8 * g) = lo;
+                *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v + 1) * G::YT_H + 8 * g) = hi;
+            }
+        }
+        __syncthreads();
+        if (l < CHUNK) Yt[f * G::YT_H + l] = to16<E>(rbuf[l]);
+    }
+    __syncthreads();
+}
+
 // Accumulates the lower tiles of the augmented Gram of one row into acc
 // (q = lane/16, c = lane%16): diagonal tiles in the MFMA accumulator layout,
 // acc[tile_id(I,I)][r] = G[16I + 4q + r][16I + c]; off-diagonal tiles as
@@ -101,44 +196,14 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                                          f32x4 (&acc)[Geo<T, E>::NT], char *lds) {
     using G = Geo<T, E>;
     const int l = lane();
-    const int f = a.f;
-    int *ibuf = reinterpret_cast<int *>(lds);
-    float *rbuf = reinterpret_cast<float *>(lds) + 32;
     char *stage = lds + G::MISC_FLOATS * 4;
-
 #pragma unroll
     for (int t = 0; t < G::NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    if constexpr (E == PREC_FP32) {
-        float *Ys = reinterpret_cast<float *>(stage);
-        const int nseg = (f + 3) >> 2;          // float4 segments carrying data
-        // Columns past the loaded segments are zero (the solve reuses this
-        // region, so clear them for every row).
-        for (int e = l; e < CHUNK * (G::FP - 4 * nseg); e += WAVE) {
-            int k = e / (G::FP - 4 * nseg), col = 4 * nseg + e % (G::FP - 4 * nseg);
-            Ys[k * G::YR + col] = 0.f;
-        }
-        const int dk = WAVE / nseg, ds = WAVE % nseg;
-        for (int k0 = 0; k0 < n; k0 += CHUNK) {
-            const int nv = min(CHUNK, n - k0);
-            if (l < CHUNK) {
-                ibuf[l] = l < nv ? a.idx[rs + k0 + l] : -1;
-                rbuf[l] = l < nv ? a.val[rs + k0 + l] : 0.f;
-            }
-            __syncthreads();
-            int k = l / nseg, s = l % nseg;
-            for (int task = l; task < CHUNK * nseg; task += WAVE) {
-                const int id = ibuf[k];
-                const float *src = id >= 0 ? a.Y + (int64_t)id * G::FP + 4 * s : a.zero32 + 4 * s;
-                const f32x4 v = *reinterpret_cast<const f32x4 *>(src);
-                *reinterpret_cast<f32x4 *>(Ys + k * G::YR + 4 * s) = v;
-                k += dk;
-                s += ds;
-                if (s >= nseg) { s -= nseg; ++k; }
-            }
-            __syncthreads();
-            if (l < CHUNK) Ys[l * G::YR + f] = rbuf[l];
-            __syncthreads();
+    stage_clear<T, E>(a.f, stage);
+    for (int k0 = 0; k0 < n; k0 += CHUNK) {
+        stage_chunk<T, E>(a, rs, k0, min(CHUNK, n - k0), lds);
+        if constexpr (E == PREC_FP32) {
+            const float *Ys = reinterpret_cast<const float *>(stage);
 #pragma unroll
             for (int ks = 0; ks < CHUNK / 4; ++ks) {
                 float F[T];
@@ -151,53 +216,8 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                         acc[tile_id(I, J)] =
                             __builtin_amdgcn_mfma_f32_16x16x4f32(F[J], F[I], acc[tile_id(I, J)], 0, 0, 0);
             }
-            __syncthreads();
-        }
-    } else {
-        uint16_t *Yt = reinterpret_cast<uint16_t *>(stage);  // Yt[factor][rating]
-        const uint16_t *Ysh = a.Ys;
-        const int nseg = (f + 7) >> 3;          // 16-byte segments carrying data
-        for (int e = l; e < (G::FP - 8 * nseg) * CHUNK; e += WAVE) {
-            int j = 8 * nseg + e / CHUNK, k = e % CHUNK;
-            Yt[j * G::YT_H + k] = 0;
-        }
-        for (int k0 = 0; k0 < n; k0 += CHUNK) {
-            const int nv = min(CHUNK, n - k0);
-            if (l < CHUNK) {
-                ibuf[l] = l < nv ? a.idx[rs + k0 + l] : -1;
-                rbuf[l] = l < nv ? a.val[rs + k0 + l] : 0.f;
-            }
-            __syncthreads();
-            // Task = (group of 8 ratings g, segment s of 8 factors): eight
-            // 16-byte loads, an 8x8 transpose of 16-bit values in registers,
-            // eight 16-byte LDS writes of Yt[8s+e][8g..8g+7].
-            for (int task = l; task < 4 * nseg; task += WAVE) {
-                const int g = task / nseg, s = task - g * nseg;
-                u32x4 d[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    // Unused slots of the last chunk read a row of zeros:
-                    // the load itself is unconditional.
-                    const int id = ibuf[8 * g + e];
-                    const uint16_t *src = id >= 0 ? Ysh + (int64_t)id * G::FP + 8 * s : a.zero16 + 8 * s;
-                    d[e] = *reinterpret_cast<const u32x4 *>(src);
-                }
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    u32x4 lo, hi;
-#pragma unroll
-                    for (int w = 0; w < 4; ++w) {
-                        const uint32_t x0 = d[2 * w][v], x1 = d[2 * w + 1][v];
-                        lo[w] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);  // low halves of x0, x1
-                        hi[w] = __builtin_amdgcn_perm(x1, x0, 0x07060302u);  // high halves
-                    }
-                    *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v) * G::YT_H + 8 * g) = lo;
-                    *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v + 1) * G::YT_H + 8 * g) = hi;
-                }
-            }
-            __syncthreads();
-            if (l < CHUNK) Yt[f * G::YT_H + l] = to16<E>(rbuf[l]);
-            __syncthreads();
+        } else {
+            const uint16_t *Yt = reinterpret_cast<const uint16_t *>(stage);
             u32x4 F[T];
 #pragma unroll
             for (int I = 0; I < T; ++I)
@@ -215,8 +235,8 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                             __builtin_bit_cast(f16x8, F[J]), __builtin_bit_cast(f16x8, F[I]), acc[tile_id(I, J)],
                             0, 0, 0);
                 }
-            __syncthreads();
         }
+        __syncthreads();
     }
 }
 
@@ -257,7 +277,7 @@ __device__ __forceinline__ double dpp_mov_d(double v) {
 // Lane N of this lane's row (DPP row_newbcast).
 template <int N>
 __device__ __forceinline__ float row_bcast(float v) {
-    return dpp_mov<0x150 + N>(v);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + N, 0xf, 0xf, true));
 }
 
 // Sum over the 16 lanes of a row. Quad sums, then mirror and rotate-by-8
@@ -276,6 +296,17 @@ __device__ __forceinline__ double row16_sum_d(double v) {
     v += dpp_mov_d<0x140>(v);
     v += dpp_mov_d<0x128>(v);
     return v;
+}
+
+// Row G's value at this lane's column, in every row. permlane16_swap(v, v)
+// leaves the even row of each row pair in the first result and the odd row
+// in the second; permlane32_swap(t, t) does the same for the two halves.
+template <int G>
+__device__ __forceinline__ float rows_bcast(float v) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    const unsigned int t = (G & 1) ? p[1] : p[0];
+    const auto s = __builtin_amdgcn_permlane32_swap(t, t, false, false);
+    return __uint_as_float((G & 2) ? s[1] : s[0]);
 }
 
 // Sum over the 4 rows: permlane16 swap pairs rows (0,1), (2,3); permlane32
@@ -356,42 +387,50 @@ __device__ __forceinline__ void extract_b(const f32x4 (&a)[NT], int f, float (&b
 // ------------------------------------------------------------- Cholesky
 // Step J of the diagonal-block factorisation: pivot J of D (standard
 // layout, kept symmetric), its column scaled by 1/sqrt(pivot), the rank-1
-// update of the trailing part, and the matching step of W = L^-1 built by
+// update of the columns past J, and the matching step of W = L^-1 built by
 // forward elimination (W[J] /= L[J][J]; W[c] -= L[c][J] W[J] for c > J),
 // with W held row by lane.
-template <int J>
-__device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, int jend, bool &ok) {
+// The rank-1 update is applied to every row of the columns past J: rows of
+// finished pivots i <= J hold the upper-triangle entries D[i][c], which the
+// same symmetric update drives to (rounding-level) zero and which no later
+// step reads for a result. That leaves one multiply-add per register.
+template <int J, bool FULL>
+__device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, int jend, float &pmin) {
     if constexpr (J < 16) {
-        if (J < jend) {
-            const int l = lane(), q = l >> 4, c = l & 15;
+        if (FULL || J < jend) {
+            int c = lane() & 15;
+            // Opaque per step: the two lane masks below are recomputed here
+            // instead of being hoisted for all 16 steps at once (which runs
+            // out of scalar registers).
+            asm volatile("" : "+v"(c));
             constexpr int JG = J >> 2, JR = J & 3;
             const float piv = bcast_lane(d[JR], (JG << 4) | J);
-            const bool good = piv > 0.f;
-            ok = ok && good;
-            const float rs = good ? __builtin_amdgcn_rsqf(piv) : 0.f;
-            const float lc = __shfl(d[JR], (JG << 4) | c, WAVE) * rs;  // L[c][J] = D[J][c] / sqrt(piv)
+            pmin = fminf(pmin, piv);  // a pivot <= 0 fails the row (its x is discarded)
+            const float rs = __builtin_amdgcn_rsqf(piv);
+            const float lc = rows_bcast<JG>(d[JR]) * rs;  // L[c][J] = D[J][c] / sqrt(piv)
+            const float coef = (c > J) ? -lc * rs : 0.f;  // -L[c][J] / sqrt(piv), zero for c <= J
+            const float sw = (c == J) ? rs : 1.f;         // row J of W scales by 1 / L[J][J]
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float li = row_bcast<J>(d[r]) * rs;  // L[4q+r][J]
-                const float wj = row_bcast<J>(w[r]) * rs;  // W[J][4q+r] / L[J][J]
-                d[r] = (4 * q + r > J && c > J) ? d[r] - li * lc : d[r];
-                w[r] = (c == J) ? wj : ((c > J) ? w[r] - lc * wj : w[r]);
+                d[r] = fmaf(row_bcast<J>(d[r]), coef, d[r]);      // -= D[4q+r][J] L[c][J] / sqrt(piv)
+                w[r] = fmaf(row_bcast<J>(w[r]), coef, w[r] * sw);  // -= L[c][J] W[J][4q+r] / L[J][J]
             }
         }
-        diag_steps<J + 1>(d, w, jend, ok);
+        diag_steps<J + 1, FULL>(d, w, jend, pmin);
     }
 }
 
-// Factor the diagonal block with jend pivots; on return the tile holds W
-// row by lane, rows >= jend zero.
-__device__ __forceinline__ void diag_factor(f32x4 &d, int jend, bool &ok) {
+// Factor the diagonal block with jend pivots (FULL: jend == 16); on return
+// the tile holds W row by lane, rows >= jend zero.
+template <bool FULL>
+__device__ __forceinline__ void diag_factor(f32x4 &d, int jend, float &pmin) {
     const int l = lane(), q = l >> 4, c = l & 15;
     f32x4 w;
 #pragma unroll
     for (int r = 0; r < 4; ++r) w[r] = (c == 4 * q + r) ? 1.f : 0.f;
-    diag_steps<0>(d, w, jend, ok);
+    diag_steps<0, FULL>(d, w, jend, pmin);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) d[r] = (c < jend) ? w[r] : 0.f;
+    for (int r = 0; r < 4; ++r) d[r] = (FULL || c < jend) ? w[r] : 0.f;
 }
 
 // Blocked right-looking Cholesky, panel K:
@@ -401,7 +440,7 @@ __device__ __forceinline__ void diag_factor(f32x4 &d, int jend, bool &ok) {
 // The last block holds row f (the augmented row, never a pivot): yl gets
 // its reduced entries (row form) just before that block is factored.
 template <int T, int K, int NT>
-__device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, bool &ok, f32x4 &yl) {
+__device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, float &pmin, f32x4 &yl) {
     if constexpr (K < T) {
         const int jend = min(16, f - 16 * K);
         if constexpr (K == T - 1) {
@@ -409,7 +448,7 @@ __device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, bool &ok, f32
 #pragma unroll
             for (int r = 0; r < 4; ++r) yl[r] = __shfl(a[tile_id(K, K)][r], (l & 48) | (f & 15), WAVE);
         }
-        diag_factor(a[tile_id(K, K)], jend, ok);
+        diag_factor<(K + 1 < T)>(a[tile_id(K, K)], jend, pmin);
         if constexpr (K + 1 < T) {
 #pragma unroll
             for (int I = K + 1; I < T; ++I) {
@@ -431,17 +470,17 @@ __device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, bool &ok, f32
                             __builtin_amdgcn_mfma_f32_16x16x4f32(nf, a[tile_id(I, K)][r], a[tile_id(I, J)], 0, 0, 0);
                 }
         }
-        chol_panels<T, K + 1>(a, f, ok, yl);
+        chol_panels<T, K + 1>(a, f, pmin, yl);
     }
 }
 
 // Factorises the augmented matrix in place and returns z = L^-1 b in row
-// form (zero past f). ok becomes false if a pivot was not positive.
+// form (zero past f); false if a pivot was not positive.
 template <int T, int NT>
 __device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, f32x4 (&zr)[T]) {
-    bool ok = true;
+    float pmin = __builtin_inff();
     f32x4 yl;
-    chol_panels<T, 0>(a, f, ok, yl);
+    chol_panels<T, 0>(a, f, pmin, yl);
     const int l = lane(), q = l >> 4, c = l & 15;
 #pragma unroll
     for (int K = 0; K + 1 < T; ++K)
@@ -454,7 +493,7 @@ __device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, f32x4 (&zr)[T])
     for (int r = 0; r < 4; ++r) u += a[tile_id(T - 1, T - 1)][r] * ((4 * q + r < jend) ? yl[r] : 0.f);
     const float zc = q4_sum(u);
     zr[T - 1] = col_to_row(c < jend ? zc : 0.f);
-    return ok;
+    return pmin > 0.f;
 }
 
 // -------------------------------------------------- triangular solves
@@ -505,17 +544,34 @@ __device__ __forceinline__ void fwd_solve(const f32x4 (&a)[NT], int f, const flo
 }
 
 // ------------------------------------------------------ refinement
-// One step of iterative refinement, x += A^-1 (b - A x): the residual is
-// formed in FP64 from Ag, this wave's scratch copy of the augmented tiles
-// taken before the factorisation (row f holds b); the correction is solved
-// on the FP32 factor in the registers.
-template <int T, int NT>
-__device__ __forceinline__ void refine_step(const f32x4 (&a)[NT], int f, const float *Ag, float (&xc)[T]) {
-    const int l = lane(), q = l >> 4, c = l & 15;
+// Iterative refinement, x += A^-1 (b - A x): the residual in FP64, the
+// correction solved on the FP32 factor in the registers.
+
+template <int G>
+__device__ __forceinline__ double rows_bcast_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned int lo = __float_as_uint(rows_bcast<G>(__uint_as_float((unsigned int)b)));
+    const unsigned int hi = __float_as_uint(rows_bcast<G>(__uint_as_float((unsigned int)(b >> 32))));
+    return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
+// Sum of the two 32-lane halves, in every lane.
+__device__ __forceinline__ double halves_sum_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned int)b, (unsigned int)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned int)(b >> 32), (unsigned int)(b >> 32), false, false);
+    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) + __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+
+// Residual from this wave's scratch copy Ag of the augmented tiles, taken
+// before the factorisation (row f holds b). x column form; r column form
+// (zero past f).
+template <int T>
+__device__ __forceinline__ void residual_copy(const float *Ag, int f, const float (&xc)[T], float (&rc)[T]) {
+    const int l = lane(), q = l >> 4;
     f32x4 xr[T];
 #pragma unroll
     for (int K = 0; K < T; ++K) xr[K] = col_to_row(xc[K]);
-    float rc[T];
 #pragma unroll
     for (int K = 0; K < T; ++K) {
         // Keep this block's scratch loads from being hoisted into earlier
@@ -549,13 +605,108 @@ __device__ __forceinline__ void refine_step(const f32x4 (&a)[NT], int f, const f
         }
         rc[K] = row_to_col(res);
     }
+}
+
+template <int T, int E>
+__device__ __forceinline__ float stage_at(const char *stage, int k, int j) {
+    using G = Geo<T, E>;
+    if constexpr (E == PREC_FP32)
+        return reinterpret_cast<const float *>(stage)[k * G::YR + j];
+    else
+        return from16<E>(reinterpret_cast<const uint16_t *>(stage)[j * G::YT_H + k]);
+}
+
+// r_j += sum_k y_k[j] e_k over the staged chunk, for factor j.
+template <int T, int E>
+__device__ __forceinline__ double stage_row_dot(const char *stage, int j, const double *eb) {
+    using G = Geo<T, E>;
+    double acc = 0.0;
+    if constexpr (E == PREC_FP32) {
+        const float *Ys = reinterpret_cast<const float *>(stage);
+#pragma unroll 8
+        for (int k = 0; k < CHUNK; ++k) acc += (double)Ys[k * G::YR + j] * eb[k];
+    } else {
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint16_t *>(stage) + j * G::YT_H);
+#pragma unroll
+        for (int v = 0; v < CHUNK / 8; ++v) {
+            const u32x4 w = p[v];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                acc += (double)from16<E>((uint16_t)(w[e] & 0xffffu)) * eb[8 * v + 2 * e];
+                acc += (double)from16<E>((uint16_t)(w[e] >> 16)) * eb[8 * v + 2 * e + 1];
+            }
+        }
+    }
+    return acc;
+}
+
+template <int T, int J, int NJ>
+__device__ __forceinline__ void lanes_to_colform(const double (&acc)[NJ], int f, float reg, const float (&xc)[T],
+                                                 float (&rc)[T]) {
+    if constexpr (J < T) {
+        // r[16J + c] sits in register J/4 of lane 16 (J % 4) + c
+        const double v = rows_bcast_d<(J & 3)>(acc[J >> 2]);
+        const int i = 16 * J + (lane() & 15);
+        rc[J] = i < f ? (float)(v - (double)reg * (double)xc[J]) : 0.f;
+        lanes_to_colform<T, J + 1>(acc, f, reg, xc, rc);
+    }
+}
+
+// Residual of one whole row without A: r = sum_i y_i (r_i - y_i . x) - reg x,
+// the row's ratings restaged 32 at a time, y_i and r_i exactly as the Gram
+// saw them (16-bit rounded on the mixed path), reg the FP32 lambda * n the
+// diagonal got. Products and sums in FP64. x column form; r column form
+// (zero past f).
+template <int T, int E>
+__device__ __forceinline__ void residual_mf(const UpdateArgs &a, int64_t rs, int n, float reg, const float (&xc)[T],
+                                            float (&rc)[T], char *lds) {
+    using G = Geo<T, E>;
+    constexpr int NJ = (G::FP + WAVE - 1) / WAVE;
+    constexpr int HALF = G::FP / 2;
+    const int l = lane(), f = a.f;
+    const float *rbuf = reinterpret_cast<const float *>(lds) + 32;
+    float *xs = reinterpret_cast<float *>(lds) + 64;
+    double *eb = reinterpret_cast<double *>(lds + (64 + G::FP) * 4);
+    const char *stage = lds + G::MISC_FLOATS * 4;
+    if (l < 16) {
+#pragma unroll
+        for (int J = 0; J < T; ++J) xs[16 * J + l] = xc[J];
+    }
+    // (the first barrier of stage_chunk orders these writes before the reads)
+    const int k = l & (CHUNK - 1), jb = (l >> 5) * HALF, je = min(f, jb + HALF);
+    double acc[NJ];
+#pragma unroll
+    for (int m = 0; m < NJ; ++m) acc[m] = 0.0;
+    for (int k0 = 0; k0 < n; k0 += CHUNK) {
+        const int nv = min(CHUNK, n - k0);
+        stage_chunk<T, E>(a, rs, k0, nv, lds);
+        // e_k = r_k - y_k . x: lane (k, half) sums the factors [jb, je)
+        double dot = 0.0;
+        for (int j = jb; j < je; ++j) dot += (double)stage_at<T, E>(stage, k, j) * (double)xs[j];
+        dot = halves_sum_d(dot);
+        if (l < CHUNK) {
+            float rk = rbuf[k];
+            if constexpr (E != PREC_FP32) rk = from16<E>(to16<E>(rk));
+            eb[k] = k < nv ? (double)rk - dot : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NJ; ++m)
+            if (WAVE * m + l < f) acc[m] += stage_row_dot<T, E>(stage, WAVE * m + l, eb);
+        __syncthreads();
+    }
+    lanes_to_colform<T, 0>(acc, f, reg, xc, rc);
+}
+
+// x += A^-1 r on the factor in the registers.
+template <int T, int NT>
+__device__ __forceinline__ void correct(const f32x4 (&a)[NT], int f, const float (&rc)[T], float (&xc)[T]) {
     f32x4 wr[T];
     fwd_solve<T>(a, f, rc, wr);
     float dc[T];
     back_solve<T>(a, f, wr, dc);
 #pragma unroll
     for (int K = 0; K < T; ++K) xc[K] += dc[K];
-    (void)c;
 }
 
 __device__ __forceinline__ float colform_dot(const float *a, const float *b, int T) {
@@ -723,18 +874,30 @@ __device__ __forceinline__ void update_body(const UpdateArgs &a) {
                     if (l == 0) atomicExch(a.counters + w.w, 0);
                 }
                 const float lam = a.lambda_rows ? a.lambda_rows[row] : a.lambda;
-                add_diag<T>(acc, f, lam * (float)n);
+                const float reg = lam * (float)n;
+                add_diag<T>(acc, f, reg);
                 bool ok = true;
                 if constexpr (S == SOLVE_CHOL || S == SOLVE_CHOL_REFINE) {
+                    // Refinement residual: restage a whole row when that moves
+                    // fewer bytes than a round trip of the tile copy.
+                    const bool restage = w.w < 0 && n * G::FP * 2 <= G::NT * 2048;
                     if constexpr (S == SOLVE_CHOL_REFINE) {
+                        if (!restage) {
 #pragma unroll
-                        for (int t = 0; t < G::NT; ++t) *reinterpret_cast<f32x4 *>(Ascr + t * 256 + 4 * l) = acc[t];
+                            for (int t = 0; t < G::NT; ++t)
+                                *reinterpret_cast<f32x4 *>(Ascr + t * 256 + 4 * l) = acc[t];
+                        }
                     }
                     f32x4 zr[T];
                     ok = chol_regs<T>(acc, f, zr);
                     back_solve<T>(acc, f, zr, x);
                     if constexpr (S == SOLVE_CHOL_REFINE) {
-                        for (int s = 0; s < a.refine; ++s) refine_step<T>(acc, f, Ascr, x);
+                        for (int s = 0; s < a.refine; ++s) {
+                            float rc[T];
+                            if (restage) residual_mf<T, E>(a, w.y, n, reg, x, rc, lds);
+                            else residual_copy<T>(Ascr, f, x, rc);
+                            correct<T>(acc, f, rc, x);
+                        }
                     }
                     if (a.Lsave) {
                         float *dst = a.Lsave + (int64_t)row * G::NT * 256;
