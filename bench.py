@@ -48,13 +48,20 @@ def parse_traffic(kernel_tag: str):
     return None
 
 
-def time_path(torch, E, ca, d, cfg, steps, warmup, tag):
+def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True):
     """Runs warmup + timed sweeps; returns (seconds, per-half kernel ms, rmse, info)."""
     stream = torch.cuda.Stream()          # default priority; a real (non-null) stream handle
     torch.cuda.set_stream(stream)
+    t = time.perf_counter()
     eng = E.Engine(cfg, d)
+    torch.cuda.synchronize()
+    upload_s = time.perf_counter() - t
     eng.set_stream(stream.cuda_stream)
     eng.init_factors()
+    tune = None
+    if autotune:
+        tune = eng.autotune()
+        log(f"[{tag}] autotune: {tune}")
     for _ in range(warmup):
         eng.sweep()
     torch.cuda.synchronize()
@@ -75,7 +82,7 @@ def time_path(torch, E, ca, d, cfg, steps, warmup, tag):
     info = eng.info()
     log(f"[{tag}] {steps} sweeps in {dt:.3f}s -> {steps / dt:.2f} sweeps/s; users {users_ms:.3f} ms, "
         f"items {items_ms:.3f} ms per half; test RMSE {rmse:.6f}; failures {info.failures}")
-    return eng, dt, users_ms, items_ms, rmse, info
+    return eng, dt, users_ms, items_ms, rmse, info, upload_s, tune
 
 
 def cpu_baseline(d, f, lam, budget_s=20.0):
@@ -139,6 +146,7 @@ def main() -> int:
     ap.add_argument("--refine", type=int, default=1)
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the cost-model schedule")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (debug only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -164,12 +172,14 @@ def main() -> int:
     prec = E.Prec.FP16 if a.prec == "fp16" else E.Prec.BF16
 
     mixed_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine)
-    eng_m, dt_m, um, im, rmse_m, info_m = time_path(torch, E, ca, d, mixed_cfg, a.steps, a.warmup, "mixed")
+    eng_m, dt_m, um, im, rmse_m, info_m, up_m, tune_m = time_path(torch, E, ca, d, mixed_cfg, a.steps, a.warmup,
+                                                                  "mixed", not a.no_autotune)
     del eng_m
     fp32 = None
     if not a.no_fp32:
         fp32_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, E.Prec.FP32)
-        eng_f, dt_f, uf, itf, rmse_f, info_f = time_path(torch, E, ca, d, fp32_cfg, a.steps, a.warmup, "fp32")
+        eng_f, dt_f, uf, itf, rmse_f, info_f, _, _ = time_path(torch, E, ca, d, fp32_cfg, a.steps, a.warmup, "fp32",
+                                                               not a.no_autotune)
         del eng_f
         fp32 = {"sweeps_per_s": a.steps / dt_f, "users_ms": uf, "items_ms": itf, "test_rmse": rmse_f,
                 "failures": int(info_f.failures)}
@@ -190,7 +200,9 @@ def main() -> int:
         "config": {"workload": f"netflix-shape {d.m}x{d.n}, {d.nnz} train ratings, f={f}, lambda={lam}",
                    "gram": f"{a.prec} MFMA, fp32 accumulate", "solve": f"fp32 Cholesky + {a.refine} fp64-residual refinement",
                    "users_ms": um, "items_ms": im, "test_rmse_after_steps": rmse_m,
-                   "failures": int(info_m.failures), "grid": list(info_m.grid), "lds_bytes": info_m.lds_bytes},
+                   "failures": int(info_m.failures), "grid": list(info_m.grid), "lds_bytes": info_m.lds_bytes,
+                   "schedule": tune_m, "h2d_upload_s": up_m,
+                   "sweeps_per_s_incl_h2d": a.steps / (dt_m + up_m)},
         "fp32_path": fp32,
         "speedup_vs_fp32": (a.steps / dt_m) / fp32["sweeps_per_s"] if fp32 else None,
         "test_rmse_delta_vs_fp32": abs(rmse_m - fp32["test_rmse"]) if fp32 else None,

@@ -51,6 +51,11 @@ class ScheduleStats(C.Structure):
                 ("split_len", C.c_int64), ("batch_work", C.c_int64)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("split_len", C.c_int64 * 2), ("batch_work", C.c_int64 * 2), ("default_ms", C.c_float * 2),
+                ("best_ms", C.c_float * 2), ("candidates", C.c_int * 2)]
+
+
 class SynthSpec(C.Structure):
     _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("total", C.c_int64), ("n_test", C.c_int64),
                 ("row_model", C.c_int), ("ln_mu", C.c_double), ("ln_sigma", C.c_double),
@@ -91,6 +96,7 @@ SIGNATURES = {
     "als_adjoint_dev": (C.c_int, [P, C.c_int, P, P, P, P, P, P]),
     "als_schedule_preview": (C.c_int, [C.c_int64, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(ScheduleStats), P, C.c_int64]),
+    "als_autotune": (C.c_int, [P, C.c_int, C.POINTER(TuneResult)]),
     # als_data.h
     "als_synth_generate": (C.c_int, [C.POINTER(SynthSpec), C.POINTER(P)]),
     "als_dataset_from_csr": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_int64, P, P, P, C.POINTER(P)]),

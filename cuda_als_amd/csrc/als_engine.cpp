@@ -67,6 +67,7 @@ struct als_engine {
     uint16_t *tmp16 = nullptr;         // shadow of caller-provided Y (forward_dev)
     size_t tmp16_bytes = 0;
     hipGraphExec_t graph = nullptr;
+    std::vector<int> hptr[2];          // host copies of the CSR / CSC row pointers (schedules)
     std::vector<void *> allocs;
     bool have_train = false, have_test = false;
 };
@@ -116,9 +117,19 @@ int check_csr(als_engine *e, int64_t rows, int64_t cols, int64_t nnz, const int 
     return ALS_OK;
 }
 
-int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr) {
-    Schedule sc = build_schedule(rows, ptr, e->cfg.split_len, e->cfg.batch_work, s.grid, e->prec);
+// (Re)build one side's work schedule and upload it; replaces any previous one.
+int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work) {
+    Schedule sc = build_schedule(rows, ptr, (int)split_len, (int)batch_work, s.grid, e->prec);
     static_assert(sizeof(Unit) == sizeof(int4) && sizeof(Split) == sizeof(int4), "unit layout");
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    dev_free(e, s.units);
+    dev_free(e, s.batch);
+    dev_free(e, s.splits);
+    dev_free(e, s.counters);
+    dev_free(e, s.partial);
+    s.units = s.splits = nullptr;
+    s.batch = s.counters = nullptr;
+    s.partial = nullptr;
     int rc;
     if ((rc = dev_alloc(e, &s.units, sc.units.size())) || (rc = dev_alloc(e, &s.batch, sc.batch_begin.size())) ||
         (rc = dev_alloc(e, &s.splits, sc.splits.size())) || (rc = dev_alloc(e, &s.counters, sc.splits.size())) ||
@@ -409,7 +420,11 @@ int als_set_train(als_engine *e, int64_t m, int64_t n, int64_t nnz, const int *c
     HIPCHK(e, hipMemsetAsync(e->V, 0, (size_t)std::max<int64_t>(1, n * e->ld) * sizeof(float), e->stream));
     if (e->Us) HIPCHK(e, hipMemsetAsync(e->Us, 0, (size_t)std::max<int64_t>(1, m * e->ld) * 2, e->stream));
     if (e->Vs) HIPCHK(e, hipMemsetAsync(e->Vs, 0, (size_t)std::max<int64_t>(1, n * e->ld) * 2, e->stream));
-    if ((rc = build_side(e, su, m, csr_ptr)) || (rc = build_side(e, si, n, csc_ptr))) return rc;
+    e->hptr[0].assign(csr_ptr, csr_ptr + m + 1);
+    e->hptr[1].assign(csc_ptr, csc_ptr + n + 1);
+    if ((rc = build_side(e, su, m, csr_ptr, e->cfg.split_len, e->cfg.batch_work)) ||
+        (rc = build_side(e, si, n, csc_ptr, e->cfg.split_len, e->cfg.batch_work)))
+        return rc;
     e->have_train = true;
     return ALS_OK;
 }
@@ -679,6 +694,96 @@ int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, c
 }
 
 }  // extern "C"
+
+// Autotuner: times one half-sweep per candidate schedule (split length x
+// batch size, around the cost-model defaults) on the engine's own data and
+// keeps the fastest. The side's factors and the failure count are restored
+// after every trial, so tuning leaves the model state unchanged.
+extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sides & ~3) return set_err(e, ALS_E_INVALID, "sides: bit 0 users, bit 1 items");
+    als_tune_result res{};
+    drop_graph(e);
+    int rc = ALS_OK;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    HIPCHK(e, hipEventCreate(&ev[0]));
+    HIPCHK(e, hipEventCreate(&ev[1]));
+    for (int sd = 0; sd < 2 && !rc; sd++) {
+        Side &s = e->side[sd];
+        if (!(sides & (1 << sd)) || s.rows == 0) {
+            res.split_len[sd] = s.split_len;
+            res.batch_work[sd] = s.batch_work;
+            continue;
+        }
+        const int64_t rows = s.rows;
+        const int *hp = e->hptr[sd].data();
+        int64_t maxrow = 1;
+        for (int64_t r = 0; r < rows; r++) maxrow = std::max<int64_t>(maxrow, hp[r + 1] - hp[r]);
+        const int64_t sl0 = s.split_len, bw0 = s.batch_work;
+        std::vector<std::pair<int64_t, int64_t>> cand;
+        for (int64_t sl : {sl0, sl0 / 2, sl0 * 2, sl0 * 4, maxrow + 1})
+            for (int64_t bw : {bw0, bw0 / 4, bw0 * 4}) {
+                const auto c = std::make_pair(std::max<int64_t>(256, std::min<int64_t>(sl, INT32_MAX / 2)),
+                                              std::max<int64_t>(1, std::min<int64_t>(bw, INT32_MAX / 2)));
+                if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+            }
+        // save what a trial overwrites
+        float *X = sd == 0 ? e->U : e->V;
+        uint16_t *Xs = sd == 0 ? e->Us : e->Vs;
+        const size_t cnt = (size_t)std::max<int64_t>(1, rows * e->ld);
+        float *xsave = nullptr;
+        uint16_t *xssave = nullptr;
+        int *fsave = nullptr;
+        if ((rc = dev_alloc(e, &xsave, cnt)) || (Xs && (rc = dev_alloc(e, &xssave, cnt))) ||
+            (rc = dev_alloc(e, &fsave, 1)))
+            break;
+        HIPCHK(e, hipMemcpyAsync(xsave, X, cnt * sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+        if (Xs) HIPCHK(e, hipMemcpyAsync(xssave, Xs, cnt * 2, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(e, hipMemcpyAsync(fsave, e->fail, sizeof(int), hipMemcpyDeviceToDevice, e->stream));
+        auto restore = [&]() -> int {
+            HIPCHK(e, hipMemcpyAsync(X, xsave, cnt * sizeof(float), hipMemcpyDeviceToDevice, e->stream));
+            if (Xs) HIPCHK(e, hipMemcpyAsync(Xs, xssave, cnt * 2, hipMemcpyDeviceToDevice, e->stream));
+            HIPCHK(e, hipMemcpyAsync(e->fail, fsave, sizeof(int), hipMemcpyDeviceToDevice, e->stream));
+            return ALS_OK;
+        };
+        float best = 1e30f;
+        size_t besti = 0;
+        for (size_t i = 0; i < cand.size() && !rc; i++) {
+            if ((rc = build_side(e, s, rows, hp, cand[i].first, cand[i].second))) break;
+            float t = 1e30f;
+            for (int rep = 0; rep < 3 && !rc; rep++) {  // first run warms caches, two timed
+                if ((rc = restore())) break;
+                HIPCHK(e, hipEventRecord(ev[0], e->stream));
+                if ((rc = update_side(e, sd))) break;
+                HIPCHK(e, hipEventRecord(ev[1], e->stream));
+                HIPCHK(e, hipEventSynchronize(ev[1]));
+                float ms = 0.f;
+                HIPCHK(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
+                if (rep > 0) t = std::min(t, ms);
+            }
+            if (i == 0) res.default_ms[sd] = t;
+            if (t < best) {
+                best = t;
+                besti = i;
+            }
+        }
+        if (!rc) rc = build_side(e, s, rows, hp, cand[besti].first, cand[besti].second);
+        if (!rc) rc = restore();
+        if (!rc) HIPCHK(e, hipStreamSynchronize(e->stream));
+        dev_free(e, xsave);
+        dev_free(e, xssave);
+        dev_free(e, fsave);
+        res.best_ms[sd] = best;
+        res.candidates[sd] = (int)cand.size();
+        res.split_len[sd] = s.split_len;
+        res.batch_work[sd] = s.batch_work;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (out) *out = res;
+    return rc;
+}
 
 extern "C" int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_work, int grid,
                                     int gram_prec, als_schedule_stats *out, int *units_out, int64_t max_units) {

@@ -135,6 +135,14 @@ class Engine:
         self._check(lib.als_get_info(self.h, C.byref(i)))
         return i
 
+    def autotune(self, sides: int = 3) -> dict:
+        """Times candidate schedules for the selected sides (bit 0 users, bit 1
+        items) and keeps the fastest; factors are left unchanged."""
+        r = A.TuneResult()
+        self._check(lib.als_autotune(self.h, sides, C.byref(r)))
+        return {"split_len": list(r.split_len), "batch_work": list(r.batch_work),
+                "default_ms": list(r.default_ms), "best_ms": list(r.best_ms), "candidates": list(r.candidates)}
+
     def device_factors(self, side: int) -> int:
         p = C.c_void_p()
         self._check(lib.als_device_factors(self.h, side, C.byref(p)))

@@ -287,3 +287,35 @@ def test_split_rows_are_deterministic(toy):
     U1, V1, r1, _ = _run_split(toy, 32, E.Prec.FP16, 5, sweeps=3)
     U2, V2, r2, _ = _run_split(toy, 32, E.Prec.FP16, 5, sweeps=3)
     assert np.array_equal(U1, U2) and np.array_equal(V1, V2) and np.array_equal(r1, r2)
+
+
+def test_autotune_keeps_state_and_results():
+    """The autotuner times candidate schedules on the engine's own data: the
+    factors it leaves behind are the ones it found, the chosen schedule is
+    the fastest candidate (the default is one of them), and a sweep on the
+    tuned schedule matches a sweep on the default one."""
+    d = ca.scaled(ca.netflix_shape(), 0.04).generate()
+    f = 32
+    cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.FP16, refine=1)
+    with E.Engine(cfg, d) as e0, E.Engine(cfg, d) as e1:
+        e0.init_factors()
+        e1.init_factors()
+        e1.sweep()
+        U_before, V_before = e1.get_factors(0), e1.get_factors(1)
+        r = e1.autotune()
+        print(r)
+        assert np.array_equal(e1.get_factors(0), U_before) and np.array_equal(e1.get_factors(1), V_before)
+        for sd in (0, 1):
+            assert r["candidates"][sd] >= 2
+            assert r["best_ms"][sd] <= r["default_ms"][sd]
+            assert r["split_len"][sd] >= 256 and r["batch_work"][sd] >= 1
+        assert list(e1.info().split_len) == r["split_len"]
+        e0.sweep()
+        e0.sweep()
+        e1.sweep()
+        U0, V0, U1, V1 = e0.get_factors(0), e0.get_factors(1), e1.get_factors(0), e1.get_factors(1)
+        assert e1.info().failures == 0
+        # split rows sum their parts in a fixed order, but a different split
+        # length cuts different parts: equal to FP32 rounding
+        assert np.max(np.abs(U0 - U1)) <= 1e-5 * max(1.0, np.abs(U0).max())
+        assert np.max(np.abs(V0 - V1)) <= 1e-5 * max(1.0, np.abs(V0).max())
