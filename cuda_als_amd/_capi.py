@@ -97,6 +97,11 @@ SIGNATURES = {
     "als_schedule_preview": (C.c_int, [C.c_int64, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(ScheduleStats), P, C.c_int64]),
     "als_autotune": (C.c_int, [P, C.c_int, C.POINTER(TuneResult)]),
+    "als_group_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
+    "als_group_destroy": (None, [P]),
+    "als_group_last_error": (C.c_char_p, [P]),
+    "als_group_update": (C.c_int, [P, C.c_int]),
+    "als_group_sweep": (C.c_int, [P]),
     # als_data.h
     "als_synth_generate": (C.c_int, [C.POINTER(SynthSpec), C.POINTER(P)]),
     "als_dataset_from_csr": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_int64, P, P, P, C.POINTER(P)]),

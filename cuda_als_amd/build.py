@@ -57,9 +57,10 @@ def build_lib() -> Path:
     jobs: list[list[str]] = []
     objs: list[Path] = []
     for p in (0, 1, 2):
-        o = OBJ / f"als_update_p{p}.o"
-        jobs.append([HIPCC, *HIP_FLAGS, f"-DALS_PREC={p}", "-c", str(CSRC / "als_update.hip"), "-o", str(o)])
-        objs.append(o)
+        for unit in ("als_update", "als_group"):
+            o = OBJ / f"{unit}_p{p}.o"
+            jobs.append([HIPCC, *HIP_FLAGS, f"-DALS_PREC={p}", "-c", str(CSRC / f"{unit}.hip"), "-o", str(o)])
+            objs.append(o)
     for src in ("als_kernels.hip", "als_engine.cpp", "als_sched.cpp"):
         o = OBJ / (Path(src).stem + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []

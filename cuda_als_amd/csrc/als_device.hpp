@@ -825,20 +825,35 @@ __device__ __forceinline__ void load_colform(const float *src, int f, float (&v)
 }
 
 // ------------------------------------------------------------- the kernel
+// Next batch index from a device work queue ([0] next batch), uniform.
+__device__ __forceinline__ int pop_batch(int *queue) {
+    int b = 0;
+    if (lane() == 0) b = atomicAdd(queue, 1);
+    return __builtin_amdgcn_readfirstlane(__shfl(b, 0, WAVE));
+}
+
+// Exit protocol: the last wave out resets the queue ([1] counts waves that
+// left) for the next launch, so a replayed graph needs no memset.
+__device__ __forceinline__ void leave_queue(int *queue) {
+    if (lane() == 0) {
+        __threadfence();
+        const int done = atomicAdd(queue + 1, 1);
+        if (done == (int)gridDim.x - 1) {
+            atomicExch(queue, 0);
+            atomicExch(queue + 1, 0);
+        }
+    }
+}
+
+// Every work unit of queue batch b of one model.
 template <int T, int E, int S>
-__device__ __forceinline__ void update_body(const UpdateArgs &a) {
+__device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds) {
     using G = Geo<T, E>;
-    extern __shared__ __attribute__((aligned(16))) char lds[];
     const int l = lane();
     const int f = a.f;
     float *Ascr = a.scratch ? a.scratch + (int64_t)blockIdx.x * G::NT * 256 : nullptr;
     f32x4 acc[G::NT];
-
-    for (;;) {
-        int b = 0;
-        if (l == 0) b = atomicAdd(a.queue, 1);
-        b = __builtin_amdgcn_readfirstlane(__shfl(b, 0, WAVE));
-        if (b >= a.nbatches) break;
+    {
         const int u0 = a.batch_begin[b], u1 = a.batch_begin[b + 1];
         for (int u = u0; u < u1; ++u) {
             const int4 w = a.units[u];
@@ -932,15 +947,17 @@ __device__ __forceinline__ void update_body(const UpdateArgs &a) {
             }
         }
     }
-    // Exit protocol: the last wave out resets the queue for the next launch.
-    if (l == 0) {
-        __threadfence();
-        const int done = atomicAdd(a.queue + 1, 1);
-        if (done == (int)gridDim.x - 1) {
-            atomicExch(a.queue, 0);
-            atomicExch(a.queue + 1, 0);
-        }
+}
+
+template <int T, int E, int S>
+__device__ __forceinline__ void update_body(const UpdateArgs &a) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    for (;;) {
+        const int b = pop_batch(a.queue);
+        if (b >= a.nbatches) break;
+        run_batch<T, E, S>(a, b, lds);
     }
+    leave_queue(a.queue);
 }
 
 template <int T, int E, int S>

@@ -160,10 +160,10 @@ int drop_graph(als_engine *e) {
     return ALS_OK;
 }
 
-// Launch one half-sweep. X/Y/shadow pointers and lambda are explicit so the
-// same path serves the engine's own factors and caller buffers.
-int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
-                const float *lambda_rows, float *Lsave) {
+// Argument block of one half-sweep. X/Y/shadow pointers and lambda are
+// explicit so the same path serves the engine's own factors and caller buffers.
+UpdateArgs side_args(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
+                     const float *lambda_rows, float *Lsave) {
     Side &s = e->side[sd];
     UpdateArgs a{};
     a.ptr = s.ptr;
@@ -192,7 +192,15 @@ int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float
     a.scratch = e->scratch;
     a.zero32 = e->zero32;
     a.zero16 = e->zero16;
+    return a;
+}
+
+// Launch one half-sweep.
+int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
+                const float *lambda_rows, float *Lsave) {
+    const Side &s = e->side[sd];
     if (s.rows == 0 || s.nbatches == 0) return ALS_OK;
+    const UpdateArgs a = side_args(e, sd, Y, Ys, X, Xs, lambda_rows, Lsave);
     HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, e->stream));
     return ALS_OK;
 }
@@ -783,6 +791,115 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
     (void)hipEventDestroy(ev[1]);
     if (out) *out = res;
     return rc;
+}
+
+// ------------------------------------------------------------------ groups
+// Several engines (models of different f) over one rating matrix, updated
+// by one persistent launch per half-sweep.
+struct als_group {
+    std::vector<als_engine *> eng;  // ordered by tile count, largest first
+    int prec = PREC_FP32, solve = SOLVE_CHOL, grid = 0, lds = 0;
+    int *queue = nullptr;
+    std::vector<float *> scratch;   // per model, refinement tile copies for the group grid
+    std::string err;
+};
+
+namespace {
+int gset_err(als_group *g, int code, const std::string &msg) {
+    g->err = msg;
+    return code;
+}
+void group_free(als_group *g) {
+    if (!g) return;
+    if (!g->eng.empty()) (void)hipSetDevice(g->eng[0]->cfg.device);
+    (void)hipFree(g->queue);
+    for (float *p : g->scratch) (void)hipFree(p);
+    delete g;
+}
+}  // namespace
+
+extern "C" int als_group_create(als_engine *const *engines, int count, als_group **out) {
+    if (!engines || !out || count < 1 || count > MAX_MODELS) return ALS_E_INVALID;
+    *out = nullptr;
+    als_engine *e0 = engines[0];
+    for (int i = 0; i < count; i++) {
+        const als_engine *e = engines[i];
+        if (!e || !e->have_train) return ALS_E_STATE;
+        if (e->m != e0->m || e->n != e0->n || e->nnz != e0->nnz) return ALS_E_INVALID;
+        if (e->prec != e0->prec || e->cfg.device != e0->cfg.device) return ALS_E_INVALID;
+        if (e->solve == SOLVE_CG || e->T < 2) return ALS_E_UNSUPPORTED;  // group kernels: Cholesky, f >= 16
+        for (int j = 0; j < i; j++)
+            if (engines[j] == e) return ALS_E_INVALID;
+    }
+    auto *g = new als_group;
+    g->eng.assign(engines, engines + count);
+    std::stable_sort(g->eng.begin(), g->eng.end(), [](const als_engine *a, const als_engine *b) { return a->T > b->T; });
+    g->prec = e0->prec;
+    g->solve = SOLVE_CHOL;
+    for (const als_engine *e : g->eng)
+        if (e->solve == SOLVE_CHOL_REFINE) g->solve = SOLVE_CHOL_REFINE;
+    for (const als_engine *e : g->eng) g->lds = std::max(g->lds, e->lds);
+    if (hipSetDevice(e0->cfg.device) != hipSuccess) { group_free(g); return ALS_E_HIP; }
+    const void *k = update_group_kernel_ptr(g->prec, g->solve);
+    int per_cu = 0;
+    if (!k || (g->lds > 65536 && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, g->lds) != hipSuccess) ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, g->lds) != hipSuccess) {
+        group_free(g);
+        return ALS_E_HIP;
+    }
+    g->grid = std::max(1, per_cu) * e0->num_cu;
+    if (hipMalloc(&g->queue, 2 * sizeof(int)) != hipSuccess ||
+        hipMemsetAsync(g->queue, 0, 2 * sizeof(int), e0->stream) != hipSuccess) {
+        group_free(g);
+        return ALS_E_HIP;
+    }
+    if (g->solve == SOLVE_CHOL_REFINE) {
+        for (const als_engine *e : g->eng) {
+            float *p = nullptr;
+            if (hipMalloc(&p, (size_t)g->grid * (e->T * (e->T + 1) / 2) * 256 * sizeof(float)) != hipSuccess) {
+                group_free(g);
+                return ALS_E_HIP;
+            }
+            g->scratch.push_back(p);
+        }
+    }
+    if (hipStreamSynchronize(e0->stream) != hipSuccess) { group_free(g); return ALS_E_HIP; }
+    *out = g;
+    return ALS_OK;
+}
+
+extern "C" void als_group_destroy(als_group *g) { group_free(g); }
+
+extern "C" const char *als_group_last_error(const als_group *g) { return g ? g->err.c_str() : "null group"; }
+
+extern "C" int als_group_update(als_group *g, int sd) {
+    if (!g) return ALS_E_INVALID;
+    if (sd != ALS_SIDE_USERS && sd != ALS_SIDE_ITEMS) return gset_err(g, ALS_E_INVALID, "side must be users or items");
+    als_engine *e0 = g->eng[0];
+    GroupArgs ga{};
+    ga.nmodels = (int)g->eng.size();
+    ga.queue = g->queue;
+    ga.first[0] = 0;
+    for (int k = 0; k < ga.nmodels; k++) {
+        als_engine *e = g->eng[k];
+        drop_graph(e);
+        UpdateArgs a = sd == ALS_SIDE_USERS ? side_args(e, 0, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr)
+                                            : side_args(e, 1, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr);
+        a.refine = e->solve == SOLVE_CHOL_REFINE ? e->cfg.refine_steps : 0;
+        a.scratch = g->scratch.empty() ? nullptr : g->scratch[k];
+        ga.m[k] = a;
+        ga.T[k] = e->T;
+        ga.first[k + 1] = ga.first[k] + (e->side[sd].rows > 0 ? e->side[sd].nbatches : 0);
+    }
+    if (ga.first[ga.nmodels] == 0) return ALS_OK;
+    const hipError_t st = launch_update_group(g->prec, g->solve, ga, g->grid, g->lds, e0->stream);
+    if (st != hipSuccess) return gset_err(g, ALS_E_HIP, std::string("group launch: ") + hipGetErrorString(st));
+    return ALS_OK;
+}
+
+extern "C" int als_group_sweep(als_group *g) {
+    int rc = als_group_update(g, ALS_SIDE_USERS);
+    return rc ? rc : als_group_update(g, ALS_SIDE_ITEMS);
 }
 
 extern "C" int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_work, int grid,

@@ -39,6 +39,17 @@ struct UpdateArgs {
     const uint16_t *zero16;     // same, 16-bit
 };
 
+// Several models (different f) on one rating matrix, one launch: batch b of
+// the combined queue is batch b - first[k] of model k (first[k] <= b < first[k+1]).
+constexpr int MAX_MODELS = 4;
+struct GroupArgs {
+    UpdateArgs m[MAX_MODELS];
+    int T[MAX_MODELS];
+    int first[MAX_MODELS + 1];
+    int nmodels;
+    int *queue;                 // [0] next batch, [1] waves that left
+};
+
 struct AdjointArgs {
     const int *ptr;             // CSR of the side that was updated
     const float *X;             // forward solution, stride 16*T
@@ -74,6 +85,8 @@ int update_lds_bytes(int T, int prec, int solve, int refine);
 hipError_t launch_update(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds_bytes,
                          hipStream_t s);
 const void *update_kernel_ptr(int T, int prec, int solve);
+hipError_t launch_update_group(int prec, int solve, const GroupArgs &g, int grid, int lds_bytes, hipStream_t s);
+const void *update_group_kernel_ptr(int prec, int solve);
 hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s);
 hipError_t launch_ybar(int T, const YbarArgs &a, hipStream_t s);
 hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, hipStream_t s);

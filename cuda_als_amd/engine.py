@@ -162,6 +162,45 @@ class Engine:
                                         C.c_void_p(d_xbar), C.c_void_p(d_lbar), C.c_void_p(d_ybar or 0)))
 
 
+class Group:
+    """Several engines (models of different f) over one rating matrix,
+    updated by one kernel launch per half-sweep (als_group_*). The engines
+    stay usable on their own and must outlive the group."""
+
+    def __init__(self, engines: list[Engine]):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        rc = lib.als_group_create(arr, len(self.engines), C.byref(h))
+        if rc != 0:
+            raise AlsError(rc, "als_group_create failed (same data, precision and device; Cholesky; f >= 16)")
+        self.h = h
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise AlsError(rc, lib.als_group_last_error(self.h).decode())
+
+    def update(self, side: int) -> None:
+        self._check(lib.als_group_update(self.h, side))
+
+    def sweep(self) -> None:
+        self._check(lib.als_group_sweep(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.als_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class AlsModel:
     """Mirror of the reference's als_model (als_model.h:29-43)."""
 

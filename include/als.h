@@ -150,6 +150,20 @@ typedef struct {
 } als_tune_result;
 int als_autotune(als_engine *e, int sides, als_tune_result *out);
 
+/* Model groups (no reference counterpart; the reference trains one model per
+ * run): up to 4 engines with different f over the same rating matrix (same
+ * m, n, nnz, Gram precision and device; Cholesky solvers; f >= 16) updated by ONE
+ * persistent kernel launch per half-sweep on the first engine's stream.
+ * Each model's result equals its own als_update exactly (same work units,
+ * same per-row arithmetic). The engines stay owned by the caller and must
+ * outlive the group. */
+typedef struct als_group als_group;
+int als_group_create(als_engine *const *engines, int count, als_group **out);
+void als_group_destroy(als_group *g);
+const char *als_group_last_error(const als_group *g);
+int als_group_update(als_group *g, int side);
+int als_group_sweep(als_group *g);
+
 /* Device factor matrices of the engine (row stride als_info.ld floats). */
 int als_device_factors(als_engine *e, int side, float **d_ptr);
 

@@ -319,3 +319,42 @@ def test_autotune_keeps_state_and_results():
         # length cuts different parts: equal to FP32 rounding
         assert np.max(np.abs(U0 - U1)) <= 1e-5 * max(1.0, np.abs(U0).max())
         assert np.max(np.abs(V0 - V1)) <= 1e-5 * max(1.0, np.abs(V0).max())
+
+
+@pytest.mark.parametrize("prec,refine", [(E.Prec.BF16, 0), (E.Prec.FP16, 1)])
+def test_group_single_launch_equals_separate_runs(prec, refine):
+    """Three models, f = 32 / 64 / 128, on one ml10M-shaped matrix (scaled),
+    updated by ONE launch per half-sweep: every model's factors equal its own
+    separate run bit for bit."""
+    d = ca.scaled(ca.ml10m_shape(), 0.1).generate()
+    fs = (32, 64, 128)
+    cfgs = [E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=refine) for f in fs]
+    sep = [E.Engine(c, d) for c in cfgs]
+    grp = [E.Engine(c, d) for c in cfgs]
+    for e in sep + grp:
+        e.init_factors()
+    with E.Group(grp) as g:
+        for _ in range(2):
+            g.sweep()
+            for e in sep:
+                e.sweep()
+    for es, eg, f in zip(sep, grp, fs):
+        for side in (0, 1):
+            a, b = es.get_factors(side), eg.get_factors(side)
+            assert np.array_equal(a, b), (f, side, np.abs(a - b).max())
+        assert eg.info().failures == 0
+        assert abs(es.rmse() - eg.rmse()) == 0.0
+    for e in sep + grp:
+        e.close()
+
+
+def test_group_rejects_mismatched_models(toy):
+    e1 = E.Engine(E.make_config(32, prec=E.Prec.FP16), toy)
+    e2 = E.Engine(E.make_config(64, prec=E.Prec.BF16), toy)
+    e3 = E.Engine(E.make_config(8, prec=E.Prec.FP16), toy)
+    with pytest.raises(ca.AlsError):
+        E.Group([e1, e2])              # different Gram precision
+    with pytest.raises(ca.AlsError):
+        E.Group([e1, e3])              # f < 16
+    for e in (e1, e2, e3):
+        e.close()
