@@ -53,7 +53,7 @@ class ScheduleStats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("split_len", C.c_int64 * 2), ("batch_work", C.c_int64 * 2), ("default_ms", C.c_float * 2),
-                ("best_ms", C.c_float * 2), ("candidates", C.c_int * 2)]
+                ("best_ms", C.c_float * 2), ("candidates", C.c_int * 2), ("restage_max", C.c_int * 2)]
 
 
 class SynthSpec(C.Structure):

@@ -893,9 +893,9 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                 add_diag<T>(acc, f, reg);
                 bool ok = true;
                 if constexpr (S == SOLVE_CHOL || S == SOLVE_CHOL_REFINE) {
-                    // Refinement residual: restage a whole row when that moves
-                    // fewer bytes than a round trip of the tile copy.
-                    const bool restage = w.w < 0 && n * G::FP * 2 <= G::NT * 2048;
+                    // Refinement residual: restage the ratings of a whole row
+                    // up to restage_max long, else keep a copy of the tiles.
+                    const bool restage = w.w < 0 && n <= a.restage_max;
                     if constexpr (S == SOLVE_CHOL_REFINE) {
                         if (!restage) {
 #pragma unroll

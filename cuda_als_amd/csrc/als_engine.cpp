@@ -35,6 +35,7 @@ struct Side {
     int nbatches = 0, nunits = 0, nsplits = 0;
     int64_t nslots = 0, split_len = 0, batch_work = 0;
     int grid = 0;
+    int restage_max = 0;           // refinement: restage rows up to this long (else tile copy)
     float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
 };
 
@@ -187,6 +188,7 @@ UpdateArgs side_args(als_engine *e, int sd, const float *Y, const uint16_t *Ys, 
     a.cg_tol = e->cfg.cg_tol;
     a.cg_fp16_a = e->cfg.solve_type == ALS_SOLVE_CG_FP16;
     a.refine = e->solve == SOLVE_CHOL_REFINE ? e->cfg.refine_steps : 0;
+    a.restage_max = s.restage_max;
     a.fail_count = e->fail;
     a.Lsave = Lsave;
     a.scratch = e->scratch;
@@ -343,6 +345,9 @@ int als_create(const als_config *cfg, als_engine **out) {
     if (!rc) {
         per_cu = std::max(1, per_cu);
         e->side[0].grid = e->side[1].grid = per_cu * e->num_cu;
+        // restaging a row moves 2 FP bytes per rating; a round trip of the
+        // tile copy 2 NT KiB: equal at 32 (T + 1) ratings
+        e->side[0].restage_max = e->side[1].restage_max = 32 * (e->T + 1);
     }
     if (!rc && e->solve == SOLVE_CHOL_REFINE)
         rc = dev_alloc(e, &e->scratch, (size_t)e->side[0].grid * (e->T * (e->T + 1) / 2) * 256);
@@ -777,8 +782,35 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             }
         }
         if (!rc) rc = build_side(e, s, rows, hp, cand[besti].first, cand[besti].second);
+        // then the refinement's restage threshold on the chosen schedule
+        if (!rc && e->solve == SOLVE_CHOL_REFINE && e->cfg.refine_steps > 0) {
+            const int r0 = s.restage_max;
+            int bestr = r0;
+            float bestt = 1e30f;
+            for (int rm : {r0, 0, 32, 64, 128, 512, 1024, 1 << 30}) {
+                s.restage_max = rm;
+                float t = 1e30f;
+                for (int rep = 0; rep < 3 && !rc; rep++) {
+                    if ((rc = restore())) break;
+                    HIPCHK(e, hipEventRecord(ev[0], e->stream));
+                    if ((rc = update_side(e, sd))) break;
+                    HIPCHK(e, hipEventRecord(ev[1], e->stream));
+                    HIPCHK(e, hipEventSynchronize(ev[1]));
+                    float ms = 0.f;
+                    HIPCHK(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
+                    if (rep > 0) t = std::min(t, ms);
+                }
+                if (t < bestt) {
+                    bestt = t;
+                    bestr = rm;
+                }
+            }
+            s.restage_max = bestr;
+            best = std::min(best, bestt);
+        }
         if (!rc) rc = restore();
         if (!rc) HIPCHK(e, hipStreamSynchronize(e->stream));
+        res.restage_max[sd] = s.restage_max;
         dev_free(e, xsave);
         dev_free(e, xssave);
         dev_free(e, fsave);

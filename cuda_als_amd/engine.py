@@ -141,7 +141,8 @@ class Engine:
         r = A.TuneResult()
         self._check(lib.als_autotune(self.h, sides, C.byref(r)))
         return {"split_len": list(r.split_len), "batch_work": list(r.batch_work),
-                "default_ms": list(r.default_ms), "best_ms": list(r.best_ms), "candidates": list(r.candidates)}
+                "default_ms": list(r.default_ms), "best_ms": list(r.best_ms), "candidates": list(r.candidates),
+                "restage_max": list(r.restage_max)}
 
     def device_factors(self, side: int) -> int:
         p = C.c_void_p()

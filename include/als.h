@@ -139,14 +139,17 @@ int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_
 /* Autotuner (no reference counterpart: the reference launches one block per
  * row with a fixed shape). For each side selected in `sides` (bit 0 users,
  * bit 1 items) times one half-sweep per candidate schedule, split length x
- * batch size around the cost-model defaults, on the engine's data, and
- * keeps the fastest. Factors and the failure count are left as they were.
+ * batch size around the cost-model defaults, then (with refinement) the
+ * residual's restage threshold, on the engine's data, and keeps the fastest. Factors and the failure count are left as they were.
  * Call after als_set_train; results are identical in exact arithmetic for
  * every schedule (split rows sum their parts in a fixed order). */
 typedef struct {
     int64_t split_len[2], batch_work[2]; /* chosen, per side */
     float default_ms[2], best_ms[2];     /* one half-sweep: default schedule, chosen one */
     int candidates[2];
+    int restage_max[2];                  /* refinement: rows up to this long restage their
+                                            ratings for the residual, longer ones keep a
+                                            copy of the tiles (tuned with refine_steps > 0) */
 } als_tune_result;
 int als_autotune(als_engine *e, int sides, als_tune_result *out);
 
