@@ -144,6 +144,10 @@ def main() -> int:
     ap.add_argument("--f", type=int, default=100)
     ap.add_argument("--prec", choices=["fp16", "bf16"], default="fp16")
     ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--refine-warm", type=int, default=1, choices=(0, 1),
+                    help="1: the refinement starts from the previous iterate (residual formed before the "
+                         "factorisation); the classic form is timed beside it unless --no-classic")
+    ap.add_argument("--no-classic", action="store_true", help="skip timing the classic-refinement variant")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost-model schedule")
@@ -171,10 +175,20 @@ def main() -> int:
     f, lam = a.f, 0.05
     prec = E.Prec.FP16 if a.prec == "fp16" else E.Prec.BF16
 
-    mixed_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine)
+    mixed_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine,
+                              refine_warm=bool(a.refine_warm))
     eng_m, dt_m, um, im, rmse_m, info_m, up_m, tune_m = time_path(torch, E, ca, d, mixed_cfg, a.steps, a.warmup,
                                                                   "mixed", not a.no_autotune)
     del eng_m
+    classic = None
+    if a.refine_warm and a.refine > 0 and not a.no_classic:
+        c_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine)
+        eng_c, dt_c, uc, ic, rmse_c, info_c, _, _ = time_path(torch, E, ca, d, c_cfg, a.steps, a.warmup,
+                                                              "mixed-classic", not a.no_autotune)
+        del eng_c
+        classic = {"sweeps_per_s": a.steps / dt_c, "users_ms": uc, "items_ms": ic, "test_rmse": rmse_c,
+                   "failures": int(info_c.failures),
+                   "solve": f"fp32 Cholesky + {a.refine} fp64-residual refinement step(s) of the fresh solve"}
     fp32 = None
     if not a.no_fp32:
         fp32_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, E.Prec.FP32)
@@ -198,12 +212,15 @@ def main() -> int:
         "warmup": a.warmup, "ms_per_step": dt_m / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": a.prec, "data": "synthetic (seeded, seed 42)",
         "config": {"workload": f"netflix-shape {d.m}x{d.n}, {d.nnz} train ratings, f={f}, lambda={lam}",
-                   "gram": f"{a.prec} MFMA, fp32 accumulate", "solve": f"fp32 Cholesky + {a.refine} fp64-residual refinement",
+                   "gram": f"{a.prec} MFMA, fp32 accumulate",
+                   "solve": (f"fp32 Cholesky + {a.refine} fp64-residual refinement step(s)"
+                             + (", the first from the previous iterate" if a.refine_warm and a.refine else "")),
                    "users_ms": um, "items_ms": im, "test_rmse_after_steps": rmse_m,
                    "failures": int(info_m.failures), "grid": list(info_m.grid), "lds_bytes": info_m.lds_bytes,
                    "schedule": tune_m, "h2d_upload_s": up_m,
                    "sweeps_per_s_incl_h2d": a.steps / (dt_m + up_m)},
         "fp32_path": fp32,
+        "mixed_classic_refine": classic,
         "speedup_vs_fp32": (a.steps / dt_m) / fp32["sweeps_per_s"] if fp32 else None,
         "test_rmse_delta_vs_fp32": abs(rmse_m - fp32["test_rmse"]) if fp32 else None,
         "roofline": roofline,

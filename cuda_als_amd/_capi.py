@@ -36,7 +36,8 @@ class AlsConfig(C.Structure):
     _fields_ = [("f", C.c_int), ("lambda_", C.c_float), ("iters", C.c_int), ("solve_type", C.c_int),
                 ("gram_prec", C.c_int), ("cg_iters", C.c_int), ("cg_tol", C.c_float),
                 ("refine_steps", C.c_int), ("m_batches", C.c_int), ("n_batches", C.c_int),
-                ("device", C.c_int), ("batch_work", C.c_int), ("split_len", C.c_int), ("use_graph", C.c_int)]
+                ("device", C.c_int), ("batch_work", C.c_int), ("split_len", C.c_int), ("use_graph", C.c_int),
+                ("refine_warm", C.c_int)]
 
 
 class AlsInfo(C.Structure):
@@ -97,6 +98,7 @@ SIGNATURES = {
     "als_schedule_preview": (C.c_int, [C.c_int64, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(ScheduleStats), P, C.c_int64]),
     "als_autotune": (C.c_int, [P, C.c_int, C.POINTER(TuneResult)]),
+    "als_set_schedule": (C.c_int, [P, C.c_int, C.c_int64, C.c_int64, C.c_int]),
     "als_group_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
     "als_group_destroy": (None, [P]),
     "als_group_last_error": (C.c_char_p, [P]),

@@ -48,6 +48,18 @@ __device__ __forceinline__ float bcast_lane(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
+// LDS hand-off between the lanes of one wave. Every workgroup of the update
+// kernels is exactly one wave, and LDS executes a wave's operations in
+// issue order, so a later read sees an earlier write of any lane once the
+// compiler keeps them in order. No s_barrier, and no wait for outstanding
+// global loads (which __syncthreads' workgroup fence would force, draining
+// the gathers prefetched for the next chunk).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
 __device__ __forceinline__ float pick4(const f32x4 &v, int i) {
     return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
 }
@@ -131,7 +143,7 @@ __device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int
         ibuf[l] = l < nv ? a.idx[rs + k0 + l] : -1;
         rbuf[l] = l < nv ? a.val[rs + k0 + l] : 0.f;
     }
-    __syncthreads();
+    wave_sync();
     if constexpr (E == PREC_FP32) {
         float *Ys = reinterpret_cast<float *>(stage);
         const int nseg = (f + 3) >> 2;
@@ -146,7 +158,7 @@ __device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int
             s += ds;
             if (s >= nseg) { s -= nseg; ++k; }
         }
-        __syncthreads();
+        wave_sync();
         if (l < CHUNK) Ys[l * G::YR + f] = rbuf[l];
     } else {
         uint16_t *Yt = reinterpret_cast<uint16_t *>(stage);
@@ -179,10 +191,10 @@ __device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int
                 *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v + 1) * G::YT_H + 8 * g) = hi;
             }
         }
-        __syncthreads();
+        wave_sync();
         if (l < CHUNK) Yt[f * G::YT_H + l] = to16<E>(rbuf[l]);
     }
-    __syncthreads();
+    wave_sync();
 }
 
 // Accumulates the lower tiles of the augmented Gram of one row into acc
@@ -236,7 +248,7 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                             0, 0, 0);
                 }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -563,6 +575,39 @@ __device__ __forceinline__ double halves_sum_d(double v) {
     return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) + __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
 }
 
+// Residual from the augmented tiles still in the registers (before the
+// factorisation; row f holds b), products and sums in FP64. Same blocking as
+// residual_copy, with the tiles read from a instead of memory.
+template <int T, int NT>
+__device__ __forceinline__ void residual_regs(const f32x4 (&a)[NT], int f, const float (&xc)[T], float (&rc)[T]) {
+    const int l = lane(), q = l >> 4;
+    f32x4 xr[T];
+#pragma unroll
+    for (int K = 0; K < T; ++K) xr[K] = col_to_row(xc[K]);
+#pragma unroll
+    for (int K = 0; K < T; ++K) {
+        double cp = 0.0;
+#pragma unroll
+        for (int J = 0; J <= K; ++J)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cp += (double)a[tile_id(K, J)][r] * (double)xr[J][r];
+        double rp[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int I = K + 1; I < T; ++I)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rp[r] += (double)a[tile_id(I, K)][r] * (double)xc[I];
+        const double cs = q4_sum_d(cp);
+        f32x4 res;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float br = __shfl(a[tile_id(T - 1, K)][r], (l & 48) | (f & 15), WAVE);  // b[16K+4q+r]
+            const double y = col_to_row_d(cs, r) + row16_sum_d(rp[r]);
+            res[r] = (16 * K + 4 * q + r < f) ? (float)((double)br - y) : 0.f;
+        }
+        rc[K] = row_to_col(res);
+    }
+}
+
 // Residual from this wave's scratch copy Ag of the augmented tiles, taken
 // before the factorisation (row f holds b). x column form; r column form
 // (zero past f).
@@ -689,11 +734,11 @@ __device__ __forceinline__ void residual_mf(const UpdateArgs &a, int64_t rs, int
             if constexpr (E != PREC_FP32) rk = from16<E>(to16<E>(rk));
             eb[k] = k < nv ? (double)rk - dot : 0.0;
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int m = 0; m < NJ; ++m)
             if (WAVE * m + l < f) acc[m] += stage_row_dot<T, E>(stage, WAVE * m + l, eb);
-        __syncthreads();
+        wave_sync();
     }
     lanes_to_colform<T, 0>(acc, f, reg, xc, rc);
 }
@@ -896,8 +941,17 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     // Refinement residual: restage the ratings of a whole row
                     // up to restage_max long, else keep a copy of the tiles.
                     const bool restage = w.w < 0 && n <= a.restage_max;
+                    // Warm-started first step: the FP64 residual of the row's
+                    // previous value x0 (still in X) is formed from the tiles
+                    // before they are factorised, and x = x0 + A^-1 r0.
+                    const bool warm = S == SOLVE_CHOL_REFINE && a.refine_warm && a.refine > 0;
+                    float rc0[T];
                     if constexpr (S == SOLVE_CHOL_REFINE) {
-                        if (!restage) {
+                        if (warm) {
+                            load_colform<T>(a.X + (int64_t)row * G::FP, f, x);
+                            residual_regs<T>(acc, f, x, rc0);
+                        }
+                        if (!restage && a.refine > (warm ? 1 : 0)) {
 #pragma unroll
                             for (int t = 0; t < G::NT; ++t)
                                 *reinterpret_cast<f32x4 *>(Ascr + t * 256 + 4 * l) = acc[t];
@@ -905,9 +959,10 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     }
                     f32x4 zr[T];
                     ok = chol_regs<T>(acc, f, zr);
-                    back_solve<T>(acc, f, zr, x);
+                    if (warm) correct<T>(acc, f, rc0, x);
+                    else back_solve<T>(acc, f, zr, x);
                     if constexpr (S == SOLVE_CHOL_REFINE) {
-                        for (int s = 0; s < a.refine; ++s) {
+                        for (int s = warm ? 1 : 0; s < a.refine; ++s) {
                             float rc[T];
                             if (restage) residual_mf<T, E>(a, w.y, n, reg, x, rc, lds);
                             else residual_copy<T>(Ascr, f, x, rc);

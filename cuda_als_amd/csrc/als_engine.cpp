@@ -189,6 +189,7 @@ UpdateArgs side_args(als_engine *e, int sd, const float *Y, const uint16_t *Ys, 
     a.cg_fp16_a = e->cfg.solve_type == ALS_SOLVE_CG_FP16;
     a.refine = e->solve == SOLVE_CHOL_REFINE ? e->cfg.refine_steps : 0;
     a.restage_max = s.restage_max;
+    a.refine_warm = e->cfg.refine_warm;
     a.fail_count = e->fail;
     a.Lsave = Lsave;
     a.scratch = e->scratch;
@@ -707,6 +708,21 @@ int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, c
 }
 
 }  // extern "C"
+
+// Applies a schedule directly (a result of als_autotune, say); values <= 0
+// keep the current ones.
+extern "C" int als_set_schedule(als_engine *e, int sd, int64_t split_len, int64_t batch_work, int restage_max) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    Side &s = e->side[sd];
+    drop_graph(e);
+    if (restage_max >= 0) s.restage_max = restage_max;
+    if (split_len > 0 || batch_work > 0)
+        return build_side(e, s, s.rows, e->hptr[sd].data(), split_len > 0 ? split_len : s.split_len,
+                          batch_work > 0 ? batch_work : s.batch_work);
+    return ALS_OK;
+}
 
 // Autotuner: times one half-sweep per candidate schedule (split length x
 // batch size, around the cost-model defaults) on the engine's own data and

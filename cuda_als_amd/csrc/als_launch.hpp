@@ -32,6 +32,7 @@ struct UpdateArgs {
     float cg_tol;
     int cg_fp16_a;              // CG on an FP16-rounded A
     int refine;                 // FP64-residual refinement steps (Cholesky)
+    int refine_warm;            // first refinement step from the row's previous value in X
     int restage_max;            // whole rows up to this many ratings form the residual by
                                 // restaging their ratings; longer ones use the tile copy
     int *fail_count;            // rows whose solve broke down (x set to 0)

@@ -46,12 +46,14 @@ def _p(a: np.ndarray) -> C.c_void_p:
 
 def make_config(f: int, lambda_: float = 0.05, iters: int = 10, solve: int = SolveType.CHOL,
                 prec: int = Prec.FP32, cg_iters: int = 6, cg_tol: float = 1e-4, refine: int = 0,
-                use_graph: bool = False, batch_work: int = 0, split_len: int = 0, device: int = 0) -> AlsConfig:
+                use_graph: bool = False, batch_work: int = 0, split_len: int = 0, device: int = 0,
+                refine_warm: bool = False) -> AlsConfig:
     cfg = AlsConfig()
     lib.als_config_init(C.byref(cfg))
     cfg.f, cfg.lambda_, cfg.iters, cfg.solve_type, cfg.gram_prec = f, lambda_, iters, int(solve), int(prec)
     cfg.cg_iters, cfg.cg_tol, cfg.refine_steps = cg_iters, cg_tol, refine
     cfg.use_graph, cfg.batch_work, cfg.split_len, cfg.device = int(use_graph), batch_work, split_len, device
+    cfg.refine_warm = int(refine_warm)
     return cfg
 
 
@@ -143,6 +145,9 @@ class Engine:
         return {"split_len": list(r.split_len), "batch_work": list(r.batch_work),
                 "default_ms": list(r.default_ms), "best_ms": list(r.best_ms), "candidates": list(r.candidates),
                 "restage_max": list(r.restage_max)}
+
+    def set_schedule(self, side: int, split_len: int = 0, batch_work: int = 0, restage_max: int = -1) -> None:
+        self._check(lib.als_set_schedule(self.h, side, split_len, batch_work, restage_max))
 
     def device_factors(self, side: int) -> int:
         p = C.c_void_p()

@@ -73,6 +73,12 @@ typedef struct {
     int batch_work;      /* ratings per work-queue batch; 0 = chosen by the scheduler */
     int split_len;       /* rows with more ratings are split across waves; 0 = chosen */
     int use_graph;       /* replay each sweep from a captured hipGraph */
+    int refine_warm;     /* 1: the first refinement step starts from the row's previous
+                            value x0 (the factor before this half-sweep): its FP64
+                            residual b - A x0 is formed from the Gram before the
+                            factorisation and x = x0 + A^-1 (b - A x0); later steps
+                            (refine_steps > 1) refine x as usual. 0: every step
+                            refines the fresh solve. */
 } als_config;
 
 typedef struct {
@@ -152,6 +158,10 @@ typedef struct {
                                             copy of the tiles (tuned with refine_steps > 0) */
 } als_tune_result;
 int als_autotune(als_engine *e, int sides, als_tune_result *out);
+/* Applies one side's schedule directly (e.g. a stored als_autotune result):
+ * split length, batch size (<= 0: keep) and the refinement restage threshold
+ * (< 0: keep). No reference counterpart. */
+int als_set_schedule(als_engine *e, int side, int64_t split_len, int64_t batch_work, int restage_max);
 
 /* Model groups (no reference counterpart; the reference trains one model per
  * run): up to 4 engines with different f over the same rating matrix (same
