@@ -86,33 +86,44 @@ def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True):
 
 
 def cpu_baseline(d, f, lam, budget_s=20.0):
-    """FP64 oracle (Cholesky) on uniformly sampled rows of both halves,
-    scaled to a full sweep. Returns the dict for the JSON line."""
+    """FP64 oracle (Cholesky) on uniformly sampled rows of both halves, scaled
+    to a full sweep; timed with every host thread, then with one (a smaller
+    sample). Returns the dict for the JSON line."""
     import numpy as np
     from oracle import als_oracle as O
     rng = np.random.default_rng(0)
     V = rng.random((d.n, f)) * 0.2
     U = rng.random((d.m, f)) * 0.2
     cfg = O.config(f, lam, O.SOLVE_CHOL)
-    per_row = {}
-    for name, rows, ptr, idx, val, Y in (("users", d.m, d.csr_ptr, d.csr_idx, d.csr_val, V),
-                                          ("items", d.n, d.csc_ptr, d.csc_idx, d.csc_val, U)):
-        k = 64
-        spent, total_rows, total_t = 0.0, 0, 0.0
-        while spent < budget_s / 2 and total_rows < rows:
-            sel = rng.integers(0, rows, size=min(k, rows))
-            t = time.perf_counter()
-            O.update_rows(cfg, sel, ptr, idx, val, Y)
-            el = time.perf_counter() - t
-            total_t += el
-            total_rows += sel.size
-            spent += el
-            k *= 2
-        per_row[name] = (total_t / total_rows, total_rows)
-    sweep_s = per_row["users"][0] * d.m + per_row["items"][0] * d.n
-    return {"value": 1.0 / sweep_s, "unit": "sweeps/s", "cores": int(O.lib.orc_num_threads()), "kind": "port",
-            "sample": (f"FP64 oracle Cholesky update of {per_row['users'][1]} uniformly sampled users and "
-                       f"{per_row['items'][1]} sampled items, per-row time scaled to all {d.m} users + {d.n} items")}
+
+    def sweep_seconds(budget):
+        per_row = {}
+        for name, rows, ptr, idx, val, Y in (("users", d.m, d.csr_ptr, d.csr_idx, d.csr_val, V),
+                                              ("items", d.n, d.csc_ptr, d.csc_idx, d.csc_val, U)):
+            k = 64
+            spent, total_rows, total_t = 0.0, 0, 0.0
+            while spent < budget / 2 and total_rows < rows:
+                sel = rng.integers(0, rows, size=min(k, rows))
+                t = time.perf_counter()
+                O.update_rows(cfg, sel, ptr, idx, val, Y)
+                el = time.perf_counter() - t
+                total_t += el
+                total_rows += sel.size
+                spent += el
+                k *= 2
+            per_row[name] = (total_t / total_rows, total_rows)
+        return per_row["users"][0] * d.m + per_row["items"][0] * d.n, per_row
+
+    cores = int(O.lib.orc_num_threads())
+    s_all, pr = sweep_seconds(budget_s)
+    O.lib.orc_set_num_threads(1)
+    s_one, pr1 = sweep_seconds(budget_s / 2)
+    O.lib.orc_set_num_threads(cores)
+    return {"value": 1.0 / s_all, "unit": "sweeps/s", "cores": cores, "kind": "port",
+            "value_1core": 1.0 / s_one,
+            "sample": (f"FP64 oracle Cholesky update of {pr['users'][1]} uniformly sampled users and "
+                       f"{pr['items'][1]} sampled items ({cores} threads; {pr1['users'][1]} + {pr1['items'][1]} "
+                       f"at 1 thread), per-row time scaled to all {d.m} users + {d.n} items")}
 
 
 def copy_calibration(torch, nbytes: int, steps: int) -> int:

@@ -579,7 +579,8 @@ __device__ __forceinline__ double halves_sum_d(double v) {
 // factorisation; row f holds b), products and sums in FP64. Same blocking as
 // residual_copy, with the tiles read from a instead of memory.
 template <int T, int NT>
-__device__ __forceinline__ void residual_regs(const f32x4 (&a)[NT], int f, const float (&xc)[T], float (&rc)[T]) {
+__device__ __forceinline__ void residual_regs(const f32x4 (&a)[NT], int f, const float (&xc)[T], float (&rc)[T],
+                                              f32x4 (&rr)[T]) {
     const int l = lane(), q = l >> 4;
     f32x4 xr[T];
 #pragma unroll
@@ -604,7 +605,27 @@ __device__ __forceinline__ void residual_regs(const f32x4 (&a)[NT], int f, const
             const double y = col_to_row_d(cs, r) + row16_sum_d(rp[r]);
             res[r] = (16 * K + 4 * q + r < f) ? (float)((double)br - y) : 0.f;
         }
+        rr[K] = res;
         rc[K] = row_to_col(res);
+    }
+}
+
+// Replaces row f of the augmented tiles (and column f of the symmetric last
+// diagonal tile) by v, given in row form vr and column form vc (zero past f):
+// the factorisation then yields L^-1 v in that row instead of L^-1 b.
+template <int T, int NT>
+__device__ __forceinline__ void set_row_f(f32x4 (&a)[NT], int f, const f32x4 (&vr)[T], const float (&vc)[T]) {
+    const int l = lane(), q = l >> 4, c = l & 15, fr = f & 15;
+#pragma unroll
+    for (int K = 0; K + 1 < T; ++K)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[tile_id(T - 1, K)][r] = (c == fr) ? vr[K][r] : a[tile_id(T - 1, K)][r];
+    constexpr int D = tile_id(T - 1, T - 1);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float v = (c == fr) ? vr[T - 1][r] : a[D][r];  // column f
+        v = (4 * q + r == fr) ? vc[T - 1] : v;          // row f
+        a[D][r] = v;
     }
 }
 
@@ -944,23 +965,31 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     // Warm-started first step: the FP64 residual of the row's
                     // previous value x0 (still in X) is formed from the tiles
                     // before they are factorised, and x = x0 + A^-1 r0.
+                    // The augmented row then carries b - A x0 instead of b, so the
+                    // factorisation itself yields L^-1 (b - A x0).
                     const bool warm = S == SOLVE_CHOL_REFINE && a.refine_warm && a.refine > 0;
-                    float rc0[T];
+                    float x0[T];
                     if constexpr (S == SOLVE_CHOL_REFINE) {
-                        if (warm) {
-                            load_colform<T>(a.X + (int64_t)row * G::FP, f, x);
-                            residual_regs<T>(acc, f, x, rc0);
-                        }
                         if (!restage && a.refine > (warm ? 1 : 0)) {
 #pragma unroll
                             for (int t = 0; t < G::NT; ++t)
                                 *reinterpret_cast<f32x4 *>(Ascr + t * 256 + 4 * l) = acc[t];
                         }
+                        if (warm) {
+                            load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
+                            float rc0[T];
+                            f32x4 rr0[T];
+                            residual_regs<T>(acc, f, x0, rc0, rr0);
+                            set_row_f<T>(acc, f, rr0, rc0);
+                        }
                     }
                     f32x4 zr[T];
                     ok = chol_regs<T>(acc, f, zr);
-                    if (warm) correct<T>(acc, f, rc0, x);
-                    else back_solve<T>(acc, f, zr, x);
+                    back_solve<T>(acc, f, zr, x);
+                    if (warm) {
+#pragma unroll
+                        for (int J = 0; J < T; ++J) x[J] += x0[J];
+                    }
                     if constexpr (S == SOLVE_CHOL_REFINE) {
                         for (int s = warm ? 1 : 0; s < a.refine; ++s) {
                             float rc[T];

@@ -875,7 +875,7 @@ extern "C" int als_group_create(als_engine *const *engines, int count, als_group
         if (!e || !e->have_train) return ALS_E_STATE;
         if (e->m != e0->m || e->n != e0->n || e->nnz != e0->nnz) return ALS_E_INVALID;
         if (e->prec != e0->prec || e->cfg.device != e0->cfg.device) return ALS_E_INVALID;
-        if (e->solve == SOLVE_CG || e->T < 2) return ALS_E_UNSUPPORTED;  // group kernels: Cholesky, f >= 16
+        if (e->solve != SOLVE_CHOL || e->T < 2) return ALS_E_UNSUPPORTED;  // group kernels: Cholesky, f >= 16
         for (int j = 0; j < i; j++)
             if (engines[j] == e) return ALS_E_INVALID;
     }
@@ -884,8 +884,6 @@ extern "C" int als_group_create(als_engine *const *engines, int count, als_group
     std::stable_sort(g->eng.begin(), g->eng.end(), [](const als_engine *a, const als_engine *b) { return a->T > b->T; });
     g->prec = e0->prec;
     g->solve = SOLVE_CHOL;
-    for (const als_engine *e : g->eng)
-        if (e->solve == SOLVE_CHOL_REFINE) g->solve = SOLVE_CHOL_REFINE;
     for (const als_engine *e : g->eng) g->lds = std::max(g->lds, e->lds);
     if (hipSetDevice(e0->cfg.device) != hipSuccess) { group_free(g); return ALS_E_HIP; }
     const void *k = update_group_kernel_ptr(g->prec, g->solve);

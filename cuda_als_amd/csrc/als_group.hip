@@ -1,9 +1,9 @@
 // One persistent launch for several models of one rating matrix (different
 // factor ranks f, so different tile counts T): the waves pop batches from a
 // combined queue and run each with the code of that model's T. Compiled once
-// per Gram precision (ALS_PREC set by the build), Cholesky solvers only, for
-// T = 2..9 (f >= 16): with the single-block body in the same function the
-// compiler's register allocation takes minutes.
+// per Gram precision (ALS_PREC set by the build), for the Cholesky solve
+// without refinement and T = 2..9 (f >= 16): with the single-block body, or
+// the refinement code, in the same function the compiler takes many minutes.
 #include "als_device.hpp"
 
 #ifndef ALS_PREC
@@ -41,9 +41,7 @@ __global__ __launch_bounds__(WAVE) void update_group_kernel(GroupArgs g) {
 #define ALS_CAT(a, b) ALS_CAT2(a, b)
 
 const void *ALS_CAT(update_group_kernel_ptr_p, ALS_PREC)(int solve) {
-    return solve == SOLVE_CHOL ? (const void *)&update_group_kernel<ALS_PREC, SOLVE_CHOL>
-           : solve == SOLVE_CHOL_REFINE ? (const void *)&update_group_kernel<ALS_PREC, SOLVE_CHOL_REFINE>
-                                        : nullptr;
+    return solve == SOLVE_CHOL ? (const void *)&update_group_kernel<ALS_PREC, SOLVE_CHOL> : nullptr;
 }
 
 hipError_t ALS_CAT(launch_update_group_p, ALS_PREC)(int solve, const GroupArgs &g, int grid, int lds,

@@ -179,7 +179,8 @@ class Group:
         h = C.c_void_p()
         rc = lib.als_group_create(arr, len(self.engines), C.byref(h))
         if rc != 0:
-            raise AlsError(rc, "als_group_create failed (same data, precision and device; Cholesky; f >= 16)")
+            raise AlsError(rc, "als_group_create failed (same data, precision and device; Cholesky without "
+                               "refinement; f >= 16)")
         self.h = h
 
     def _check(self, rc: int) -> None:

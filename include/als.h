@@ -165,7 +165,8 @@ int als_set_schedule(als_engine *e, int side, int64_t split_len, int64_t batch_w
 
 /* Model groups (no reference counterpart; the reference trains one model per
  * run): up to 4 engines with different f over the same rating matrix (same
- * m, n, nnz, Gram precision and device; Cholesky solvers; f >= 16) updated by ONE
+ * m, n, nnz, Gram precision and device; Cholesky solve without refinement;
+ * f >= 16) updated by ONE
  * persistent kernel launch per half-sweep on the first engine's stream.
  * Each model's result equals its own als_update exactly (same work units,
  * same per-row arithmetic). The engines stay owned by the caller and must

@@ -19,6 +19,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* Reference: als_model.cu:1772-1777.
  *   srand(0); for k < n*f: h_VT[k] = 0.2*((float)rand() / (float)RAND_MAX);
  *   for k < m*f: h_UT[k] = 0;

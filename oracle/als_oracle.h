@@ -115,6 +115,8 @@ void orc_adjoint(int f, int64_t rows, int64_t ncols, const int *ptr,
 
 /* Number of OpenMP threads the oracle will use. */
 int orc_num_threads(void);
+/* Sets it (n > 0); used to time the CPU baseline at one core. */
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

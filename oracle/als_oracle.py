@@ -37,6 +37,7 @@ _SIG = {
                               C.c_int, P, P, P]),
     "orc_adjoint": (None, [C.c_int, C.c_int64, C.c_int64, P, P, P, P, P, P, P, P, P, P]),
     "orc_num_threads": (C.c_int, []),
+    "orc_set_num_threads": (None, [C.c_int]),
 }
 
 

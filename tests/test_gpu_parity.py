@@ -324,7 +324,7 @@ def test_autotune_keeps_state_and_results():
         assert np.max(np.abs(V0 - V1)) <= 2e-4 * max(1.0, np.abs(V0).max())
 
 
-@pytest.mark.parametrize("prec,refine", [(E.Prec.BF16, 0), (E.Prec.FP16, 1)])
+@pytest.mark.parametrize("prec,refine", [(E.Prec.BF16, 0), (E.Prec.FP16, 0)])
 def test_group_single_launch_equals_separate_runs(prec, refine):
     """Three models, f = 32 / 64 / 128, on one ml10M-shaped matrix (scaled),
     updated by ONE launch per half-sweep: every model's factors equal its own
@@ -355,11 +355,14 @@ def test_group_rejects_mismatched_models(toy):
     e1 = E.Engine(E.make_config(32, prec=E.Prec.FP16), toy)
     e2 = E.Engine(E.make_config(64, prec=E.Prec.BF16), toy)
     e3 = E.Engine(E.make_config(8, prec=E.Prec.FP16), toy)
+    e4 = E.Engine(E.make_config(48, prec=E.Prec.FP16, refine=1), toy)
     with pytest.raises(ca.AlsError):
         E.Group([e1, e2])              # different Gram precision
     with pytest.raises(ca.AlsError):
         E.Group([e1, e3])              # f < 16
-    for e in (e1, e2, e3):
+    with pytest.raises(ca.AlsError):
+        E.Group([e1, e4])              # refinement
+    for e in (e1, e2, e3, e4):
         e.close()
 
 
