@@ -397,52 +397,54 @@ __device__ __forceinline__ void extract_b(const f32x4 (&a)[NT], int f, float (&b
 }
 
 // ------------------------------------------------------------- Cholesky
-// Step J of the diagonal-block factorisation: pivot J of D (standard
-// layout, kept symmetric), its column scaled by 1/sqrt(pivot), the rank-1
-// update of the columns past J, and the matching step of W = L^-1 built by
-// forward elimination (W[J] /= L[J][J]; W[c] -= L[c][J] W[J] for c > J),
-// with W held row by lane.
+// Step J of the diagonal-block factorisation D = L~ P L~^T (L~ unit lower,
+// P the pivots, L = L~ P^1/2). D is in the standard layout, kept symmetric;
+// the step takes pivot J, applies the rank-1 update with coefficient
+// -D[J][c] / P[J] to the columns past J, and the matching forward-elimination
+// step of W~ = L~^-1 (W~[c] -= L~[c][J] W~[J], c > J), W~ held row by lane.
+// Both updates share the coefficient, and W~'s rows need no scaling until
+// the end (W = P^-1/2 W~: row c times 1/sqrt(P[c])).
 // The rank-1 update is applied to every row of the columns past J: rows of
 // finished pivots i <= J hold the upper-triangle entries D[i][c], which the
 // same symmetric update drives to (rounding-level) zero and which no later
-// step reads for a result. That leaves one multiply-add per register.
+// step reads for a result. A pivot <= 0 makes its 1/sqrt non-finite, which
+// reaches x through W; the caller's finiteness check fails the row.
 template <int J, bool FULL>
-__device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, int jend, float &pmin) {
+__device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, float &myrs, int jend) {
     if constexpr (J < 16) {
         if (FULL || J < jend) {
             int c = lane() & 15;
-            // Opaque per step: the two lane masks below are recomputed here
+            // Opaque per step: the lane masks below are recomputed here
             // instead of being hoisted for all 16 steps at once (which runs
             // out of scalar registers).
             asm volatile("" : "+v"(c));
             constexpr int JG = J >> 2, JR = J & 3;
             const float piv = bcast_lane(d[JR], (JG << 4) | J);
-            pmin = fminf(pmin, piv);  // a pivot <= 0 fails the row (its x is discarded)
             const float rs = __builtin_amdgcn_rsqf(piv);
-            const float lc = rows_bcast<JG>(d[JR]) * rs;  // L[c][J] = D[J][c] / sqrt(piv)
-            const float coef = (c > J) ? -lc * rs : 0.f;  // -L[c][J] / sqrt(piv), zero for c <= J
-            const float sw = (c == J) ? rs : 1.f;         // row J of W scales by 1 / L[J][J]
+            const float coef = (c > J) ? -rows_bcast<JG>(d[JR]) * (rs * rs) : 0.f;  // -D[J][c] / P[J]
+            myrs = (c == J) ? rs : myrs;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                d[r] = fmaf(row_bcast<J>(d[r]), coef, d[r]);      // -= D[4q+r][J] L[c][J] / sqrt(piv)
-                w[r] = fmaf(row_bcast<J>(w[r]), coef, w[r] * sw);  // -= L[c][J] W[J][4q+r] / L[J][J]
+                d[r] = fmaf(row_bcast<J>(d[r]), coef, d[r]);  // -= D[4q+r][J] D[J][c] / P[J]
+                w[r] = fmaf(row_bcast<J>(w[r]), coef, w[r]);  // -= L~[c][J] W~[J][4q+r]
             }
         }
-        diag_steps<J + 1, FULL>(d, w, jend, pmin);
+        diag_steps<J + 1, FULL>(d, w, myrs, jend);
     }
 }
 
 // Factor the diagonal block with jend pivots (FULL: jend == 16); on return
-// the tile holds W row by lane, rows >= jend zero.
+// the tile holds W = L^-1 row by lane, rows >= jend zero.
 template <bool FULL>
-__device__ __forceinline__ void diag_factor(f32x4 &d, int jend, float &pmin) {
+__device__ __forceinline__ void diag_factor(f32x4 &d, int jend) {
     const int l = lane(), q = l >> 4, c = l & 15;
     f32x4 w;
 #pragma unroll
     for (int r = 0; r < 4; ++r) w[r] = (c == 4 * q + r) ? 1.f : 0.f;
-    diag_steps<0, FULL>(d, w, jend, pmin);
+    float myrs = 0.f;
+    diag_steps<0, FULL>(d, w, myrs, jend);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) d[r] = (FULL || c < jend) ? w[r] : 0.f;
+    for (int r = 0; r < 4; ++r) d[r] = (FULL || c < jend) ? w[r] * myrs : 0.f;
 }
 
 // Blocked right-looking Cholesky, panel K:
@@ -452,7 +454,7 @@ __device__ __forceinline__ void diag_factor(f32x4 &d, int jend, float &pmin) {
 // The last block holds row f (the augmented row, never a pivot): yl gets
 // its reduced entries (row form) just before that block is factored.
 template <int T, int K, int NT>
-__device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, float &pmin, f32x4 &yl) {
+__device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, f32x4 &yl) {
     if constexpr (K < T) {
         const int jend = min(16, f - 16 * K);
         if constexpr (K == T - 1) {
@@ -460,7 +462,7 @@ __device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, float &pmin, 
 #pragma unroll
             for (int r = 0; r < 4; ++r) yl[r] = __shfl(a[tile_id(K, K)][r], (l & 48) | (f & 15), WAVE);
         }
-        diag_factor<(K + 1 < T)>(a[tile_id(K, K)], jend, pmin);
+        diag_factor<(K + 1 < T)>(a[tile_id(K, K)], jend);
         if constexpr (K + 1 < T) {
 #pragma unroll
             for (int I = K + 1; I < T; ++I) {
@@ -482,17 +484,17 @@ __device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, float &pmin, 
                             __builtin_amdgcn_mfma_f32_16x16x4f32(nf, a[tile_id(I, K)][r], a[tile_id(I, J)], 0, 0, 0);
                 }
         }
-        chol_panels<T, K + 1>(a, f, pmin, yl);
+        chol_panels<T, K + 1>(a, f, yl);
     }
 }
 
 // Factorises the augmented matrix in place and returns z = L^-1 b in row
-// form (zero past f); false if a pivot was not positive.
+// form (zero past f). A pivot <= 0 leaves non-finite values that reach x
+// (the caller checks x); the return value is kept for that interface.
 template <int T, int NT>
 __device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, f32x4 (&zr)[T]) {
-    float pmin = __builtin_inff();
     f32x4 yl;
-    chol_panels<T, 0>(a, f, pmin, yl);
+    chol_panels<T, 0>(a, f, yl);
     const int l = lane(), q = l >> 4, c = l & 15;
 #pragma unroll
     for (int K = 0; K + 1 < T; ++K)
@@ -505,7 +507,7 @@ __device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, f32x4 (&zr)[T])
     for (int r = 0; r < 4; ++r) u += a[tile_id(T - 1, T - 1)][r] * ((4 * q + r < jend) ? yl[r] : 0.f);
     const float zc = q4_sum(u);
     zr[T - 1] = col_to_row(c < jend ? zc : 0.f);
-    return pmin > 0.f;
+    return true;
 }
 
 // -------------------------------------------------- triangular solves
