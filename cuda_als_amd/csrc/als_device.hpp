@@ -577,23 +577,33 @@ __device__ __forceinline__ double halves_sum_d(double v) {
     return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) + __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
 }
 
-// Residual from the augmented tiles still in the registers (before the
-// factorisation; row f holds b), products and sums in FP64. Same blocking as
-// residual_copy, with the tiles read from a instead of memory.
+// Warm start: replaces the augmented row f (b) by r0 = b - A x0, formed in
+// FP64 from the tiles still in the registers (before the factorisation), so
+// that the factorisation yields L^-1 r0 in that row. Same blocking as
+// residual_copy with the tiles read from a. Block K's residual is written
+// into row f as soon as it is formed: row f enters the other blocks' sums
+// only through x0[f] = 0 or through the discarded entry f, and a later
+// block's b is read before its own overwrite. x0 in column form (zero past
+// f); the last diagonal tile keeps its row and column f symmetric.
+// x0 is read from xs (16T floats of this wave's LDS), where the row form
+// of a block is one 16-byte read instead of 4 registers per block.
 template <int T, int NT>
-__device__ __forceinline__ void residual_regs(const f32x4 (&a)[NT], int f, const float (&xc)[T], float (&rc)[T],
-                                              f32x4 (&rr)[T]) {
-    const int l = lane(), q = l >> 4;
-    f32x4 xr[T];
+__device__ __forceinline__ void warm_row_f(f32x4 (&a)[NT], int f, const float (&xc)[T], float *xs) {
+    const int l = lane(), q = l >> 4, c = l & 15, fr = f & 15;
+    if (l < 16) {
 #pragma unroll
-    for (int K = 0; K < T; ++K) xr[K] = col_to_row(xc[K]);
+        for (int J = 0; J < T; ++J) xs[16 * J + l] = xc[J];
+    }
+    wave_sync();
 #pragma unroll
     for (int K = 0; K < T; ++K) {
         double cp = 0.0;
 #pragma unroll
-        for (int J = 0; J <= K; ++J)
+        for (int J = 0; J <= K; ++J) {
+            const f32x4 xJ = *reinterpret_cast<const f32x4 *>(xs + 16 * J + 4 * q);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) cp += (double)a[tile_id(K, J)][r] * (double)xr[J][r];
+            for (int r = 0; r < 4; ++r) cp += (double)a[tile_id(K, J)][r] * (double)xJ[r];
+        }
         double rp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int I = K + 1; I < T; ++I)
@@ -603,31 +613,23 @@ __device__ __forceinline__ void residual_regs(const f32x4 (&a)[NT], int f, const
         f32x4 res;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float br = __shfl(a[tile_id(T - 1, K)][r], (l & 48) | (f & 15), WAVE);  // b[16K+4q+r]
+            const float br = __shfl(a[tile_id(T - 1, K)][r], (l & 48) | fr, WAVE);  // b[16K+4q+r]
             const double y = col_to_row_d(cs, r) + row16_sum_d(rp[r]);
             res[r] = (16 * K + 4 * q + r < f) ? (float)((double)br - y) : 0.f;
         }
-        rr[K] = res;
-        rc[K] = row_to_col(res);
-    }
-}
-
-// Replaces row f of the augmented tiles (and column f of the symmetric last
-// diagonal tile) by v, given in row form vr and column form vc (zero past f):
-// the factorisation then yields L^-1 v in that row instead of L^-1 b.
-template <int T, int NT>
-__device__ __forceinline__ void set_row_f(f32x4 (&a)[NT], int f, const f32x4 (&vr)[T], const float (&vc)[T]) {
-    const int l = lane(), q = l >> 4, c = l & 15, fr = f & 15;
+        if (K + 1 < T) {
 #pragma unroll
-    for (int K = 0; K + 1 < T; ++K)
+            for (int r = 0; r < 4; ++r) a[tile_id(T - 1, K)][r] = (c == fr) ? res[r] : a[tile_id(T - 1, K)][r];
+        } else {
+            const float resc = row_to_col(res);
+            constexpr int D = tile_id(T - 1, T - 1);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[tile_id(T - 1, K)][r] = (c == fr) ? vr[K][r] : a[tile_id(T - 1, K)][r];
-    constexpr int D = tile_id(T - 1, T - 1);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        float v = (c == fr) ? vr[T - 1][r] : a[D][r];  // column f
-        v = (4 * q + r == fr) ? vc[T - 1] : v;          // row f
-        a[D][r] = v;
+            for (int r = 0; r < 4; ++r) {
+                float v = (c == fr) ? res[r] : a[D][r];  // column f
+                v = (4 * q + r == fr) ? resc : v;        // row f
+                a[D][r] = v;
+            }
+        }
     }
 }
 
@@ -960,7 +962,7 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                 const float reg = lam * (float)n;
                 add_diag<T>(acc, f, reg);
                 bool ok = true;
-                if constexpr (S == SOLVE_CHOL || S == SOLVE_CHOL_REFINE) {
+                if constexpr (S == SOLVE_CHOL || S == SOLVE_CHOL_REFINE || S == SOLVE_CHOL_WARM) {
                     // Refinement residual: restage the ratings of a whole row
                     // up to restage_max long, else keep a copy of the tiles.
                     const bool restage = w.w < 0 && n <= a.restage_max;
@@ -969,7 +971,8 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     // before they are factorised, and x = x0 + A^-1 r0.
                     // The augmented row then carries b - A x0 instead of b, so the
                     // factorisation itself yields L^-1 (b - A x0).
-                    const bool warm = S == SOLVE_CHOL_REFINE && a.refine_warm && a.refine > 0;
+                    const bool warm =
+                        S == SOLVE_CHOL_WARM || (S == SOLVE_CHOL_REFINE && a.refine_warm && a.refine > 0);
                     float x0[T];
                     if constexpr (S == SOLVE_CHOL_REFINE) {
                         if (!restage && a.refine > (warm ? 1 : 0)) {
@@ -977,18 +980,19 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                             for (int t = 0; t < G::NT; ++t)
                                 *reinterpret_cast<f32x4 *>(Ascr + t * 256 + 4 * l) = acc[t];
                         }
+                    }
+                    if constexpr (S == SOLVE_CHOL_REFINE || S == SOLVE_CHOL_WARM) {
                         if (warm) {
                             load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
-                            float rc0[T];
-                            f32x4 rr0[T];
-                            residual_regs<T>(acc, f, x0, rc0, rr0);
-                            set_row_f<T>(acc, f, rr0, rc0);
+                            warm_row_f<T>(acc, f, x0, reinterpret_cast<float *>(lds) + 64);
                         }
                     }
                     f32x4 zr[T];
                     ok = chol_regs<T>(acc, f, zr);
                     back_solve<T>(acc, f, zr, x);
                     if (warm) {
+                        // x0 again from X (not kept live across the factorisation)
+                        load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
 #pragma unroll
                         for (int J = 0; J < T; ++J) x[J] += x0[J];
                     }

@@ -187,7 +187,7 @@ UpdateArgs side_args(als_engine *e, int sd, const float *Y, const uint16_t *Ys, 
     a.cg_iters = e->cfg.cg_iters;
     a.cg_tol = e->cfg.cg_tol;
     a.cg_fp16_a = e->cfg.solve_type == ALS_SOLVE_CG_FP16;
-    a.refine = e->solve == SOLVE_CHOL_REFINE ? e->cfg.refine_steps : 0;
+    a.refine = (e->solve == SOLVE_CHOL_REFINE || e->solve == SOLVE_CHOL_WARM) ? e->cfg.refine_steps : 0;
     a.restage_max = s.restage_max;
     a.refine_warm = e->cfg.refine_warm;
     a.fail_count = e->fail;
@@ -316,6 +316,7 @@ int als_create(const als_config *cfg, als_engine **out) {
     e->ld = 16 * e->T;
     e->prec = cfg->gram_prec;
     e->solve = (cfg->solve_type == ALS_SOLVE_CG_FP32 || cfg->solve_type == ALS_SOLVE_CG_FP16) ? SOLVE_CG
+               : cfg->refine_steps == 1 && cfg->refine_warm                               ? SOLVE_CHOL_WARM
                : cfg->refine_steps > 0                                                    ? SOLVE_CHOL_REFINE
                                                                                           : SOLVE_CHOL;
     int ndev = 0;

@@ -7,7 +7,9 @@
 namespace als {
 
 enum Prec { PREC_FP32 = 0, PREC_BF16 = 1, PREC_FP16 = 2 };
-enum SolveKind { SOLVE_CHOL = 0, SOLVE_CG = 1, SOLVE_CHOL_REFINE = 2 };
+// SOLVE_CHOL_WARM: one warm-started refinement step and no other (its own
+// kernel: without the classic residual code the registers fit without spills).
+enum SolveKind { SOLVE_CHOL = 0, SOLVE_CG = 1, SOLVE_CHOL_REFINE = 2, SOLVE_CHOL_WARM = 3 };
 constexpr int MAX_T = 9;  // augmented dimension f + 1 <= 16 * MAX_T
 
 struct UpdateArgs {

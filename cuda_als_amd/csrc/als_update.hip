@@ -52,9 +52,10 @@ static int lds_for(int solve, int refine) {
 #define ALS_CAT(a, b) ALS_CAT2(a, b)
 
 const void *ALS_CAT(update_kernel_ptr_p, ALS_PREC)(int T, int solve) {
-    return solve == SOLVE_CHOL ? kernel_for<SOLVE_CHOL>(T)
-           : solve == SOLVE_CG ? kernel_for<SOLVE_CG>(T)
-                               : kernel_w2_for<SOLVE_CHOL_REFINE>(T);
+    return solve == SOLVE_CHOL        ? kernel_for<SOLVE_CHOL>(T)
+           : solve == SOLVE_CG        ? kernel_for<SOLVE_CG>(T)
+           : solve == SOLVE_CHOL_WARM ? kernel_w2_for<SOLVE_CHOL_WARM>(T)
+                                      : kernel_w2_for<SOLVE_CHOL_REFINE>(T);
 }
 
 int ALS_CAT(update_lds_bytes_p, ALS_PREC)(int T, int solve, int refine) {
