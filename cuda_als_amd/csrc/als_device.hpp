@@ -421,13 +421,30 @@ __device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, float &myrs, int 
             constexpr int JG = J >> 2, JR = J & 3;
             const float piv = bcast_lane(d[JR], (JG << 4) | J);
             const float rs = __builtin_amdgcn_rsqf(piv);
-            const float coef = (c > J) ? -rows_bcast<JG>(d[JR]) * (rs * rs) : 0.f;  // -D[J][c] / P[J]
+            // The broadcast runs in every lane (not under the c > J mask), so
+            // the permlane swaps see a full wave and need no branch.
+            const float rb = rows_bcast<JG>(d[JR]);
+            const float coef = (c > J) ? -rb * (rs * rs) : 0.f;  // -D[J][c] / P[J]
             myrs = (c == J) ? rs : myrs;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                d[r] = fmaf(row_bcast<J>(d[r]), coef, d[r]);  // -= D[4q+r][J] D[J][c] / P[J]
-                w[r] = fmaf(row_bcast<J>(w[r]), coef, w[r]);  // -= L~[c][J] W~[J][4q+r]
-            }
+            // d[r] += row_bcast<J>(d[r]) * coef   (-= D[4q+r][J] D[J][c] / P[J])
+            // w[r] += row_bcast<J>(w[r]) * coef   (-= L~[c][J] W~[J][4q+r])
+            // with the broadcast folded into the FMA as its DPP source operand:
+            // one instruction per element instead of a DPP move plus an FMA.
+            // s_nop 1 gives the two wait states a DPP read needs after a VALU
+            // write of its source; inside the block no instruction reads the
+            // register its predecessor wrote.
+            asm volatile(
+                "s_nop 1\n\t"
+                "v_fmac_f32_dpp %0, %0, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f32_dpp %1, %1, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f32_dpp %2, %2, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f32_dpp %3, %3, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f32_dpp %4, %4, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f32_dpp %5, %5, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f32_dpp %6, %6, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+                "v_fmac_f32_dpp %7, %7, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf"
+                : "+v"(d[0]), "+v"(d[1]), "+v"(d[2]), "+v"(d[3]), "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3])
+                : "v"(coef), "i"(J));
         }
         diag_steps<J + 1, FULL>(d, w, myrs, jend);
     }
