@@ -257,32 +257,40 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
 // column's own square sum) is left alone: it is never a pivot.
 template <int T, int NT>
 __device__ __forceinline__ void add_diag(f32x4 (&acc)[NT], int f, float reg) {
-    const int l = lane(), q = l >> 4, c = l & 15;
+    int l = lane();
+    // Opaque per row: the lane masks stay per-row vector compares instead of
+    // being hoisted out of the row loop into scalar registers (which spill).
+    asm volatile("" : "+v"(l));
+    const int q = l >> 4, c = l & 15, rr = c & 3;
+    // Lane (q, c) holds diagonal entry c of every diagonal tile when
+    // c / 4 == q, in register c % 4.
+    const bool mine = (c >> 2) == q;
 #pragma unroll
-    for (int I = 0; I < T; ++I)
+    for (int I = 0; I < T; ++I) {
+        const int d = 16 * I + c;
+        f32x4 &t = acc[tile_id(I, I)];
+        const float v = pick4(t, rr);
+        const float nv = d < f ? v + reg : d > f ? 1.f : v;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const bool on = 4 * q + r == c;
-            const int d = 16 * I + c;
-            float v = acc[tile_id(I, I)][r];
-            v = (on && d < f) ? v + reg : v;
-            v = (on && d > f) ? 1.f : v;
-            acc[tile_id(I, I)][r] = v;
-        }
+        for (int r = 0; r < 4; ++r) t[r] = (mine && rr == r) ? nv : t[r];
+    }
 }
 
 // ------------------------------------------------------- lane exchange
 // A 64-lane wave is 4 rows (q = lane / 16) of 16 lanes (c = lane % 16).
 
+// DPP moves for patterns whose source lanes are all valid (quad_perm,
+// row_mirror, row_ror): bound_ctrl set, so no "old" value has to be
+// materialised in the destination first.
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov_d(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -410,7 +418,7 @@ __device__ __forceinline__ void extract_b(const f32x4 (&a)[NT], int f, float (&b
 // step reads for a result. A pivot <= 0 makes its 1/sqrt non-finite, which
 // reaches x through W; the caller's finiteness check fails the row.
 template <int J, bool FULL>
-__device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, float &myrs, int jend) {
+__device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, int jend) {
     if constexpr (J < 16) {
         if (FULL || J < jend) {
             int c = lane() & 15;
@@ -425,7 +433,6 @@ __device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, float &myrs, int 
             // the permlane swaps see a full wave and need no branch.
             const float rb = rows_bcast<JG>(d[JR]);
             const float coef = (c > J) ? -rb * (rs * rs) : 0.f;  // -D[J][c] / P[J]
-            myrs = (c == J) ? rs : myrs;
             // d[r] += row_bcast<J>(d[r]) * coef   (-= D[4q+r][J] D[J][c] / P[J])
             // w[r] += row_bcast<J>(w[r]) * coef   (-= L~[c][J] W~[J][4q+r])
             // with the broadcast folded into the FMA as its DPP source operand:
@@ -446,7 +453,7 @@ __device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, float &myrs, int 
                 : "+v"(d[0]), "+v"(d[1]), "+v"(d[2]), "+v"(d[3]), "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3])
                 : "v"(coef), "i"(J));
         }
-        diag_steps<J + 1, FULL>(d, w, myrs, jend);
+        diag_steps<J + 1, FULL>(d, w, jend);
     }
 }
 
@@ -458,8 +465,13 @@ __device__ __forceinline__ void diag_factor(f32x4 &d, int jend) {
     f32x4 w;
 #pragma unroll
     for (int r = 0; r < 4; ++r) w[r] = (c == 4 * q + r) ? 1.f : 0.f;
-    float myrs = 0.f;
-    diag_steps<0, FULL>(d, w, myrs, jend);
+    diag_steps<0, FULL>(d, w, jend);
+    // Row scaling 1/sqrt(P[c]). Steps past c leave column c alone (their
+    // coefficient is zero there), so D[c][c] is still the value step c
+    // pivoted on; it sits in row c/4, register c%4, and the sum over the
+    // four rows adds three zeros to it.
+    const float dc = pick4(d, c & 3);
+    const float myrs = __builtin_amdgcn_rsqf(q4_sum((q == (c >> 2)) ? dc : 0.f));
 #pragma unroll
     for (int r = 0; r < 4; ++r) d[r] = (FULL || c < jend) ? w[r] * myrs : 0.f;
 }
