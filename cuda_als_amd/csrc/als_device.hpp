@@ -606,22 +606,45 @@ __device__ __forceinline__ double halves_sum_d(double v) {
     return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) + __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
 }
 
+// Sums of four values over the 16 lanes of a row, scattered: lane c gets
+// the total of v[c & 3]. Two exchange levels inside the quad (partners c^2,
+// then c^1) halve the values carried, then the four quads are added with
+// rotations by 4 and 8, which keep c & 3. Lanes c, c+4, c+8, c+12 hold the
+// same total, possibly rounded differently; callers read one fixed lane.
+__device__ __forceinline__ double rows16_scatter_d(const double (&v)[4]) {
+    const int c = lane() & 15;
+    const bool b1 = (c & 2) != 0, b0 = (c & 1) != 0;
+    const double k0 = b1 ? v[2] : v[0], k1 = b1 ? v[3] : v[1];
+    const double s0 = b1 ? v[0] : v[2], s1 = b1 ? v[1] : v[3];
+    const double u0 = k0 + dpp_mov_d<0x4E>(s0);  // quad_perm [2,3,0,1]
+    const double u1 = k1 + dpp_mov_d<0x4E>(s1);
+    double t = (b0 ? u1 : u0) + dpp_mov_d<0xB1>(b0 ? u0 : u1);  // quad_perm [1,0,3,2]
+    t += dpp_mov_d<0x124>(t);  // row_ror:4
+    t += dpp_mov_d<0x128>(t);  // row_ror:8
+    return t;
+}
+
 // Warm start: replaces the augmented row f (b) by r0 = b - A x0, formed in
 // FP64 from the tiles still in the registers (before the factorisation), so
-// that the factorisation yields L^-1 r0 in that row. Same blocking as
-// residual_copy with the tiles read from a. Block K's residual is written
-// into row f as soon as it is formed: row f enters the other blocks' sums
-// only through x0[f] = 0 or through the discarded entry f, and a later
-// block's b is read before its own overwrite. x0 in column form (zero past
-// f); the last diagonal tile keeps its row and column f symmetric.
-// x0 is read from xs (16T floats of this wave's LDS), where the row form
-// of a block is one 16-byte read instead of 4 registers per block.
+// that the factorisation yields L^-1 r0 in that row. Block K of y = A x0:
+//   tiles (K, J <= K) give row 16K + c from lane (q, c) (summed over q),
+//   tiles (I > K, K) give row 16K + 4q + r from register r (summed over c).
+// b is copied to bs (16T floats of LDS) before any block is overwritten;
+// row f enters the other blocks' sums only through x0[f] = 0 or through the
+// discarded entry f. x0 in column form (zero past f); xs (16T floats of LDS)
+// holds it in memory order, where the row form of a block is one 16-byte
+// read. The last diagonal tile keeps its row and column f symmetric.
 template <int T, int NT>
-__device__ __forceinline__ void warm_row_f(f32x4 (&a)[NT], int f, const float (&xc)[T], float *xs) {
+__device__ __forceinline__ void warm_row_f(f32x4 (&a)[NT], int f, const float (&xc)[T], float *xs, float *bs) {
     const int l = lane(), q = l >> 4, c = l & 15, fr = f & 15;
     if (l < 16) {
 #pragma unroll
         for (int J = 0; J < T; ++J) xs[16 * J + l] = xc[J];
+    }
+    if (c == fr) {
+        // lane (q, f % 16) holds b[16K + 4q + r] in register r of tile (T-1, K)
+#pragma unroll
+        for (int K = 0; K < T; ++K) *reinterpret_cast<f32x4 *>(bs + 16 * K + 4 * q) = a[tile_id(T - 1, K)];
     }
     wave_sync();
 #pragma unroll
@@ -638,19 +661,16 @@ __device__ __forceinline__ void warm_row_f(f32x4 (&a)[NT], int f, const float (&
         for (int I = K + 1; I < T; ++I)
 #pragma unroll
             for (int r = 0; r < 4; ++r) rp[r] += (double)a[tile_id(I, K)][r] * (double)xc[I];
-        const double cs = q4_sum_d(cp);
-        f32x4 res;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const float br = __shfl(a[tile_id(T - 1, K)][r], (l & 48) | fr, WAVE);  // b[16K+4q+r]
-            const double y = col_to_row_d(cs, r) + row16_sum_d(rp[r]);
-            res[r] = (16 * K + 4 * q + r < f) ? (float)((double)br - y) : 0.f;
-        }
+        // row 16K + c: the column-form part, plus the row-form part taken from
+        // lane (c / 4, c), which holds the total for register c % 4 of row c / 4
+        const double rs = (K + 1 < T) ? rows16_scatter_d(rp) : 0.0;
+        const double y = q4_sum_d(cp) + __shfl(rs, ((c >> 2) << 4) | c, WAVE);
+        const float resc = (16 * K + c < f) ? (float)((double)bs[16 * K + c] - y) : 0.f;
+        const f32x4 res = col_to_row(resc);  // lane (q, c): r0[16K + 4q + r]
         if (K + 1 < T) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[tile_id(T - 1, K)][r] = (c == fr) ? res[r] : a[tile_id(T - 1, K)][r];
         } else {
-            const float resc = row_to_col(res);
             constexpr int D = tile_id(T - 1, T - 1);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -1013,7 +1033,8 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     if constexpr (S == SOLVE_CHOL_REFINE || S == SOLVE_CHOL_WARM) {
                         if (warm) {
                             load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
-                            warm_row_f<T>(acc, f, x0, reinterpret_cast<float *>(lds) + 64);
+                            warm_row_f<T>(acc, f, x0, reinterpret_cast<float *>(lds) + 64,
+                                          reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4));
                         }
                     }
                     f32x4 zr[T];
