@@ -91,12 +91,22 @@ template <> __device__ __forceinline__ float from16<PREC_FP16>(uint16_t h) {
 }
 
 // LDS carve-up of one wave (the workgroup is exactly one wave).
+#ifndef ALS_DUAL_TD_MAX
+#define ALS_DUAL_TD_MAX 3
+#endif
+
 template <int T, int E> struct Geo {
     static constexpr int NT = T * (T + 1) / 2;  // lower-triangle 16x16 tiles
     static constexpr int FP = 16 * T;           // padded augmented dimension
     static constexpr int YT_H = 40;             // 16-bit stage row: 32 ratings + 8 pad (80 B)
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
-    static constexpr int STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : FP * YT_H * 2;
+    // Rows with fewer than 16 * DUAL_TD ratings take the dual system (16-bit
+    // Gram, T >= 5), whose FP32 tile copy lives in the stage.
+    static constexpr int DUAL_TD =
+        (E != PREC_FP32 && T >= 5) ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
+    static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : FP * YT_H * 2;
+    static constexpr int DUAL_STAGE_BYTES = DUAL_TD * (DUAL_TD + 1) / 2 * 1024;
+    static constexpr int STAGE_BYTES = GRAM_STAGE_BYTES > DUAL_STAGE_BYTES ? GRAM_STAGE_BYTES : DUAL_STAGE_BYTES;
     // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
     static constexpr int MISC_FLOATS = 32 + 32 + FP + 64;
     __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + STAGE_BYTES; }
@@ -964,6 +974,181 @@ __device__ __forceinline__ void leave_queue(int *queue) {
     }
 }
 
+// ------------------------------------------------------------ dual solve
+// A row with n ratings, n < f, has the same solution in the n x n space of
+// its ratings. With Y the n x f matrix of the rated rows (16-bit) and
+// mu = lambda n, the push-through identity gives
+//   x = (Y^T Y + mu I)^-1 Y^T r = Y^T (Y Y^T + mu I)^-1 r,
+// so the kernel factors the augmented [[Y Y^T + mu I, r], [r^T, 0]] in
+// TD = n / 16 + 1 tiles instead of T, with the same register Cholesky.
+
+// Sums of eight values over the 16 lanes of a row, scattered: lane c gets
+// the total of v[c >> 1] (lanes c and c ^ 1 hold the same total). Exchange
+// partners c ^ 8 (row_ror:8), c ^ 7 (row_half_mirror) and c ^ 2 halve the
+// values carried; c ^ 1 adds the last pair.
+__device__ __forceinline__ float rows16_scatter8(const float (&v)[8]) {
+    const int c = lane() & 15;
+    const bool b3 = (c & 8) != 0, b2 = (c & 4) != 0, b1 = (c & 2) != 0;
+    float u[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = (b3 ? v[4 + i] : v[i]) + dpp_mov<0x128>(b3 ? v[i] : v[4 + i]);
+    float t[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) t[i] = (b2 ? u[2 + i] : u[i]) + dpp_mov<0x141>(b2 ? u[i] : u[2 + i]);
+    float z = (b1 ? t[1] : t[0]) + dpp_mov<0x4E>(b1 ? t[0] : t[1]);
+    return z + dpp_mov<0xB1>(z);
+}
+
+// Lane (q, c)'s 16-byte segment 4s + q of rated row 16I + c (zero past the
+// ratings and past the segments that carry data).
+template <int TD, int T, int E>
+__device__ __forceinline__ void dual_operands(const UpdateArgs &a, const int (&id)[TD], int s, int nseg,
+                                              u32x4 (&F)[TD]) {
+    using G = Geo<T, E>;
+    const int seg = 4 * s + (lane() >> 4);
+#pragma unroll
+    for (int I = 0; I < TD; ++I) {
+        F[I] = u32x4{0u, 0u, 0u, 0u};
+        if (id[I] >= 0 && seg < nseg) F[I] = *reinterpret_cast<const u32x4 *>(a.Ys + (int64_t)id[I] * G::FP + 8 * seg);
+    }
+}
+
+// One row through the dual system: x (column form of the T-tile layout,
+// zero past f). One refinement step: the FP64 residual r - K alpha from an
+// FP32 copy of the tiles taken before the factorisation, the correction
+// solved on the factor. xs: 16T floats of this wave's LDS.
+template <int TD, int T, int E>
+__device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n, float reg, float (&xc)[T],
+                                         float *xs, float *kst) {
+    constexpr int NTD = TD * (TD + 1) / 2;
+    static_assert(NTD * 1024 <= Geo<T, E>::STAGE_BYTES, "tile copy must fit the stage");
+    int l = lane();
+    // Opaque per row: lane masks of this path are formed here, not hoisted
+    // out of the row loop where they would stay live across the other path.
+    asm volatile("" : "+v"(l));
+    const int q = l >> 4, c = l & 15;
+    const int f = a.f, nseg = (f + 7) >> 3, ks = (nseg + 3) >> 2;
+    int id[TD];
+    float rc[TD];  // ratings, column form, rounded to the Gram precision as the augmented Gram rounds them
+#pragma unroll
+    for (int I = 0; I < TD; ++I) {
+        const int k = 16 * I + c;
+        id[I] = k < n ? a.idx[rs + k] : -1;
+        rc[I] = k < n ? from16<E>(to16<E>(a.val[rs + k])) : 0.f;
+    }
+    // K = Y Y^T: off-diagonal tiles come out row by lane (operands swapped)
+    f32x4 acc[NTD];
+#pragma unroll
+    for (int t = 0; t < NTD; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < ks; ++s) {
+        u32x4 F[TD];
+        dual_operands<TD, T, E>(a, id, s, nseg, F);
+#pragma unroll
+        for (int I = 0; I < TD; ++I)
+#pragma unroll
+            for (int J = 0; J <= I; ++J) {
+                if constexpr (E == PREC_BF16)
+                    acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, F[J]), __builtin_bit_cast(bf16x8, F[I]), acc[tile_id(I, J)], 0, 0, 0);
+                else
+                    acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(f16x8, F[J]), __builtin_bit_cast(f16x8, F[I]), acc[tile_id(I, J)], 0, 0, 0);
+            }
+    }
+    add_diag<TD>(acc, n, reg);
+    // row n <- r: row by lane in the tiles left of the diagonal (lanes
+    // c == n % 16), row and column n in the last diagonal tile
+    {
+        const int nr = n & 15;
+#pragma unroll
+        for (int J = 0; J + 1 < TD; ++J) {
+            const f32x4 rr = col_to_row(rc[J]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[tile_id(TD - 1, J)][r] = (c == nr) ? rr[r] : acc[tile_id(TD - 1, J)][r];
+        }
+        const f32x4 rr = col_to_row(rc[TD - 1]);
+        constexpr int D = tile_id(TD - 1, TD - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = (c == nr) ? rr[r] : acc[D][r];  // column n
+            v = (4 * q + r == nr) ? rc[TD - 1] : v;   // row n
+            acc[D][r] = v;
+        }
+    }
+    // FP32 copy of the tiles for the residual, in the stage (unused here)
+#pragma unroll
+    for (int t = 0; t < NTD; ++t) *reinterpret_cast<f32x4 *>(kst + t * 256 + 4 * l) = acc[t];
+    f32x4 zr[TD];
+    chol_regs<TD>(acc, n, zr);
+    float al[TD];
+    back_solve<TD>(acc, n, zr, al);
+    // refinement: e = r - K alpha in FP64 (alpha is zero from index n on, so
+    // row and column n of the copy add nothing to the rows kept)
+    if (l < 16) {
+#pragma unroll
+        for (int J = 0; J < TD; ++J) xs[16 * J + l] = al[J];
+    }
+    wave_sync();
+    float ec[TD];
+#pragma unroll
+    for (int K = 0; K < TD; ++K) {
+        double cp = 0.0;
+#pragma unroll
+        for (int J = 0; J <= K; ++J) {
+            const f32x4 aJ = *reinterpret_cast<const f32x4 *>(xs + 16 * J + 4 * q);
+            const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(K, J) * 256 + 4 * l);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cp += (double)kt[r] * (double)aJ[r];
+        }
+        double rp[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int I = K + 1; I < TD; ++I) {
+            const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(I, K) * 256 + 4 * l);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rp[r] += (double)kt[r] * (double)al[I];
+        }
+        const double rsum = (K + 1 < TD) ? rows16_scatter_d(rp) : 0.0;
+        const double y = q4_sum_d(cp) + __shfl(rsum, ((c >> 2) << 4) | c, WAVE);
+        ec[K] = (16 * K + c < n) ? (float)((double)rc[K] - y) : 0.f;
+    }
+    correct<TD>(acc, n, ec, al);
+    wave_sync();  // every lane has read alpha from xs before x is written there
+    // x = Y^T alpha: lane (q, c) multiplies segment 4s + q of row 16I + c by
+    // alpha[16I + c]; the sum over c lands on lanes c (even) for factor
+    // 8 (4s + q) + c / 2.
+    for (int s = 0; s < ks; ++s) {
+        u32x4 F[TD];
+        dual_operands<TD, T, E>(a, id, s, nseg, F);
+        float p[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int I = 0; I < TD; ++I)
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                p[2 * w] = fmaf(from16<E>((uint16_t)(F[I][w] & 0xffffu)), al[I], p[2 * w]);
+                p[2 * w + 1] = fmaf(from16<E>((uint16_t)(F[I][w] >> 16)), al[I], p[2 * w + 1]);
+            }
+        const float z = rows16_scatter8(p);
+        const int seg = 4 * s + q;
+        if ((c & 1) == 0 && seg < nseg) xs[8 * seg + (c >> 1)] = z;
+    }
+    wave_sync();
+#pragma unroll
+    for (int J = 0; J < T; ++J) xc[J] = (16 * J + c < f) ? xs[16 * J + c] : 0.f;
+    wave_sync();  // x read before the next row's use of xs
+}
+
+// The dual system of the fewest tiles that hold n + 1 rows.
+template <int TD, int TDMAX, int T, int E>
+__device__ __forceinline__ void dual_pick(const UpdateArgs &a, int64_t rs, int n, float reg, float (&xc)[T],
+                                          float *xs, float *kst) {
+    if constexpr (TD < TDMAX) {
+        if (n < 16 * TD) dual_row<TD, T, E>(a, rs, n, reg, xc, xs, kst);
+        else dual_pick<TD + 1, TDMAX, T, E>(a, rs, n, reg, xc, xs, kst);
+    } else {
+        dual_row<TD, T, E>(a, rs, n, reg, xc, xs, kst);
+    }
+}
+
 // Every work unit of queue batch b of one model.
 template <int T, int E, int S>
 __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds) {
@@ -981,7 +1166,26 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
             float x[T];
 #pragma unroll
             for (int J = 0; J < T; ++J) x[J] = 0.f;
-            if (n > 0) {
+            bool dual = false;
+            if constexpr (S == SOLVE_CHOL_WARM && G::DUAL_TD > 0) {
+                // Whole rows under 16 * DUAL_TD ratings: the dual system.
+                if (n > 0 && n < 16 * G::DUAL_TD && w.w < 0 && !a.Lsave) {
+                    const float reg = (a.lambda_rows ? a.lambda_rows[row] : a.lambda) * (float)n;
+                    float *xsd = reinterpret_cast<float *>(lds) + 64;
+                    float *kst = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);
+                    dual_pick<1, G::DUAL_TD, T, E>(a, w.y, n, reg, x, xsd, kst);
+                    dual = true;
+                    bool finite = true;
+#pragma unroll
+                    for (int J = 0; J < T; ++J) finite = finite && isfinite(x[J]);
+                    if (__any(!finite)) {
+                        if (l == 0) atomicAdd(a.fail_count, 1);
+#pragma unroll
+                        for (int J = 0; J < T; ++J) x[J] = 0.f;
+                    }
+                }
+            }
+            if (n > 0 && !dual) {
                 gram_row<T, E>(a, w.y, w.z - w.y, acc, lds);
                 if (w.w >= 0) {
                     // Split row: publish this part's tiles; the last part to

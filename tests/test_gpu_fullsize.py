@@ -6,10 +6,12 @@ checked on sampled rows from the GPU's own state (uniform users, and the
 heaviest items, which the schedule splits across waves), plus the test RMSE
 recomputed on the host from the downloaded factors.
 Tolerances (relative to max(1, |x|)): mixed path (FP16 Gram, FP32 Cholesky,
-one FP64-residual refinement step) 5e-5 against the FP64 solve of the
-FP16-rounded system: rows whose residual is formed from the restaged ratings
-land within ~3e-8, rows refined against the FP32-accumulated tile copy (long
-and split rows) keep that accumulation's error, ~1e-5 on the heaviest items.
+one FP64-residual refinement step, classic or warm-started) 5e-5 against the
+FP64 solve of the FP16-rounded system: rows whose residual is formed from the
+restaged ratings land within ~3e-8, rows refined against the FP32-accumulated
+tiles (long and split rows, every warm-started row, and the light rows solved
+through their dual system) keep that accumulation's error, ~1e-5 on the
+heaviest items.
 FP32 path 1e-4 against the exact FP64 solve.
 """
 import numpy as np
@@ -27,15 +29,19 @@ def netflix():
     return ca.netflix_shape().generate()
 
 
-@pytest.mark.parametrize("prec,refine,oprec,tol", [(E.Prec.FP16, 1, O.PREC_FP16, 5e-5),
-                                                   (E.Prec.FP32, 0, O.PREC_EXACT, 1e-4)])
-def test_sampled_rows_match_oracle_at_full_size(netflix, prec, refine, oprec, tol):
+@pytest.mark.parametrize("prec,refine,warm,oprec,tol", [(E.Prec.FP16, 1, False, O.PREC_FP16, 5e-5),
+                                                        (E.Prec.FP16, 1, True, O.PREC_FP16, 5e-5),
+                                                        (E.Prec.FP32, 0, False, O.PREC_EXACT, 1e-4)])
+def test_sampled_rows_match_oracle_at_full_size(netflix, prec, refine, warm, oprec, tol):
     d, f = netflix, 100
     rng = np.random.default_rng(11)
-    users = np.sort(rng.choice(d.m, 300, replace=False))
+    cnt_u = np.diff(d.csr_ptr)
+    # uniform users plus users light enough for the dual system
+    users = np.unique(np.concatenate([rng.choice(d.m, 300, replace=False),
+                                      rng.choice(np.where(cnt_u < 48)[0], 100, replace=False)]))
     cnt = np.diff(d.csc_ptr)
     items = np.sort(np.concatenate([np.argsort(cnt)[-40:], rng.choice(d.n, 60, replace=False)]))
-    with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=refine), d) as e:
+    with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=refine, refine_warm=warm), d) as e:
         e.init_factors()
         e.sweep()
         e.sweep()

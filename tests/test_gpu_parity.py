@@ -407,3 +407,34 @@ def test_warm_started_refinement(toy):
         p.update(ca.ALS_SIDE_USERS)
         a, b = e.get_factors(0), p.get_factors(0)
         assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("f", [64, 100])
+@pytest.mark.parametrize("prec,oprec", [(E.Prec.FP16, O.PREC_FP16), (E.Prec.BF16, O.PREC_BF16)])
+def test_dual_system_for_light_rows(toy, f, prec, oprec):
+    """With a 16-bit Gram and one warm-started refinement step, whole rows
+    with fewer than 48 ratings (f >= 64) are solved in the space of their
+    ratings, x = Y^T (Y Y^T + mu I)^-1 r. Every user of the toy workload
+    (at most 40 ratings) and most of its items take that path. One
+    half-sweep of each side from the oracle's trained state matches the
+    FP64 solve of the 16-bit-rounded system within 1e-5 relative to
+    max(1, |x|); the rows through the dual system land within ~2e-7."""
+    assert np.diff(toy.csr_ptr).max() < 48
+    U0, V0 = O.reference_init(toy.m, toy.n, f)
+    Uo, Vo, _, _ = O.train(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec, store_fp32=True), toy, 3, U0, V0)
+    with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=1, refine_warm=True), toy) as e:
+        e.set_factors(0, Uo.astype(np.float32))
+        e.set_factors(1, Vo.astype(np.float32))
+        e.update(ca.ALS_SIDE_USERS)
+        Ug = e.get_factors(0)
+        e.update(ca.ALS_SIDE_ITEMS)
+        Vg = e.get_factors(1)
+        assert e.info().failures == 0
+    cfg = O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec)
+    Un, _ = O.update(cfg, toy.csr_ptr, toy.csr_idx, toy.csr_val, Vo.astype(np.float32), Uo)
+    Vn, _ = O.update(cfg, toy.csc_ptr, toy.csc_idx, toy.csc_val, Ug.astype(np.float64), Vo)
+    eu = np.abs(Ug - Un).max(axis=1) / np.maximum(1.0, np.abs(Un).max(axis=1))
+    ev = np.abs(Vg - Vn).max(axis=1) / np.maximum(1.0, np.abs(Vn).max(axis=1))
+    light = np.diff(toy.csc_ptr) < 48
+    print("users", eu.max(), "light items", ev[light].max(), "other items", ev[~light].max())
+    assert eu.max() <= 1e-6 and ev[light].max() <= 1e-6 and ev.max() <= 1e-5
