@@ -100,10 +100,9 @@ template <int T, int E> struct Geo {
     static constexpr int FP = 16 * T;           // padded augmented dimension
     static constexpr int YT_H = 40;             // 16-bit stage row: 32 ratings + 8 pad (80 B)
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
-    // Rows with fewer than 16 * DUAL_TD ratings take the dual system (16-bit
-    // Gram, T >= 5), whose FP32 tile copy lives in the stage.
-    static constexpr int DUAL_TD =
-        (E != PREC_FP32 && T >= 5) ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
+    // Rows with fewer than 16 * DUAL_TD ratings take the dual system
+    // (T >= 5), whose FP32 tile copy lives in the stage.
+    static constexpr int DUAL_TD = T >= 5 ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
     static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : FP * YT_H * 2;
     static constexpr int DUAL_STAGE_BYTES = DUAL_TD * (DUAL_TD + 1) / 2 * 1024;
     static constexpr int STAGE_BYTES = GRAM_STAGE_BYTES > DUAL_STAGE_BYTES ? GRAM_STAGE_BYTES : DUAL_STAGE_BYTES;
@@ -999,42 +998,70 @@ __device__ __forceinline__ float rows16_scatter8(const float (&v)[8]) {
     return z + dpp_mov<0xB1>(z);
 }
 
-// Lane (q, c)'s 16-byte segment 4s + q of rated row 16I + c (zero past the
-// ratings and past the segments that carry data).
+// Sums of four values over the 16 lanes of a row, scattered: lane c gets
+// the total of v[c & 3] (the FP32 form of rows16_scatter_d).
+__device__ __forceinline__ float rows16_scatter4(const float (&v)[4]) {
+    const int c = lane() & 15;
+    const bool b1 = (c & 2) != 0, b0 = (c & 1) != 0;
+    const float u0 = (b1 ? v[2] : v[0]) + dpp_mov<0x4E>(b1 ? v[0] : v[2]);
+    const float u1 = (b1 ? v[3] : v[1]) + dpp_mov<0x4E>(b1 ? v[1] : v[3]);
+    float t = (b0 ? u1 : u0) + dpp_mov<0xB1>(b0 ? u0 : u1);
+    t += dpp_mov<0x124>(t);
+    t += dpp_mov<0x128>(t);
+    return t;
+}
+
+// Operands of the dual Gram, one step of the factor dimension: lane (q, c)
+// holds the 16-byte segment 4s + q of rated row 16I + c (zero past the
+// ratings and past the segments that carry data). 16-bit rows: 8 factors,
+// one 16x16x32 MFMA per tile. FP32 rows: 4 factors, four 16x16x4 MFMAs,
+// the j-th taking factor 4 (4s + q) + j as its k = q (the contraction over
+// the factors runs in any order, the same for both operands).
 template <int TD, int T, int E>
 __device__ __forceinline__ void dual_operands(const UpdateArgs &a, const int (&id)[TD], int s, int nseg,
                                               u32x4 (&F)[TD]) {
     using G = Geo<T, E>;
     const int seg = 4 * s + (lane() >> 4);
+    constexpr int SEGW = E == PREC_FP32 ? 4 : 8;  // elements per 16-byte segment
 #pragma unroll
     for (int I = 0; I < TD; ++I) {
         F[I] = u32x4{0u, 0u, 0u, 0u};
-        if (id[I] >= 0 && seg < nseg) F[I] = *reinterpret_cast<const u32x4 *>(a.Ys + (int64_t)id[I] * G::FP + 8 * seg);
+        if (id[I] >= 0 && seg < nseg) {
+            if constexpr (E == PREC_FP32)
+                F[I] = *reinterpret_cast<const u32x4 *>(a.Y + (int64_t)id[I] * G::FP + SEGW * seg);
+            else
+                F[I] = *reinterpret_cast<const u32x4 *>(a.Ys + (int64_t)id[I] * G::FP + SEGW * seg);
+        }
     }
 }
 
 // One row through the dual system: x (column form of the T-tile layout,
-// zero past f). One refinement step: the FP64 residual r - K alpha from an
-// FP32 copy of the tiles taken before the factorisation, the correction
-// solved on the factor. xs: 16T floats of this wave's LDS.
-template <int TD, int T, int E>
+// zero past f). With REFINE, one refinement step: the FP64 residual
+// r - K alpha from an FP32 copy of the tiles taken before the
+// factorisation (kst: NTD KB of LDS), the correction solved on the factor.
+// xs: 16T floats of this wave's LDS.
+template <int TD, int T, int E, bool REFINE>
 __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n, float reg, float (&xc)[T],
                                          float *xs, float *kst) {
     constexpr int NTD = TD * (TD + 1) / 2;
-    static_assert(NTD * 1024 <= Geo<T, E>::STAGE_BYTES, "tile copy must fit the stage");
+    static_assert(!REFINE || NTD * 1024 <= Geo<T, E>::STAGE_BYTES, "tile copy must fit the stage");
     int l = lane();
     // Opaque per row: lane masks of this path are formed here, not hoisted
     // out of the row loop where they would stay live across the other path.
     asm volatile("" : "+v"(l));
     const int q = l >> 4, c = l & 15;
-    const int f = a.f, nseg = (f + 7) >> 3, ks = (nseg + 3) >> 2;
+    const int f = a.f;
+    constexpr int SEGW = E == PREC_FP32 ? 4 : 8;
+    const int nseg = (f + SEGW - 1) / SEGW, ks = (nseg + 3) >> 2;
     int id[TD];
     float rc[TD];  // ratings, column form, rounded to the Gram precision as the augmented Gram rounds them
 #pragma unroll
     for (int I = 0; I < TD; ++I) {
         const int k = 16 * I + c;
         id[I] = k < n ? a.idx[rs + k] : -1;
-        rc[I] = k < n ? from16<E>(to16<E>(a.val[rs + k])) : 0.f;
+        float v = k < n ? a.val[rs + k] : 0.f;
+        if constexpr (E != PREC_FP32) v = from16<E>(to16<E>(v));
+        rc[I] = v;
     }
     // K = Y Y^T: off-diagonal tiles come out row by lane (operands swapped)
     f32x4 acc[NTD];
@@ -1047,12 +1074,18 @@ __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n,
         for (int I = 0; I < TD; ++I)
 #pragma unroll
             for (int J = 0; J <= I; ++J) {
-                if constexpr (E == PREC_BF16)
+                if constexpr (E == PREC_FP32) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                            __uint_as_float(F[J][j]), __uint_as_float(F[I][j]), acc[tile_id(I, J)], 0, 0, 0);
+                } else if constexpr (E == PREC_BF16) {
                     acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         __builtin_bit_cast(bf16x8, F[J]), __builtin_bit_cast(bf16x8, F[I]), acc[tile_id(I, J)], 0, 0, 0);
-                else
+                } else {
                     acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
                         __builtin_bit_cast(f16x8, F[J]), __builtin_bit_cast(f16x8, F[I]), acc[tile_id(I, J)], 0, 0, 0);
+                }
             }
     }
     add_diag<TD>(acc, n, reg);
@@ -1075,61 +1108,75 @@ __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n,
             acc[D][r] = v;
         }
     }
-    // FP32 copy of the tiles for the residual, in the stage (unused here)
+    if constexpr (REFINE) {
+        // FP32 copy of the tiles for the residual, in the stage (unused here)
 #pragma unroll
-    for (int t = 0; t < NTD; ++t) *reinterpret_cast<f32x4 *>(kst + t * 256 + 4 * l) = acc[t];
+        for (int t = 0; t < NTD; ++t) *reinterpret_cast<f32x4 *>(kst + t * 256 + 4 * l) = acc[t];
+    }
     f32x4 zr[TD];
     chol_regs<TD>(acc, n, zr);
     float al[TD];
     back_solve<TD>(acc, n, zr, al);
-    // refinement: e = r - K alpha in FP64 (alpha is zero from index n on, so
-    // row and column n of the copy add nothing to the rows kept)
-    if (l < 16) {
+    if constexpr (REFINE) {
+        // e = r - K alpha in FP64 (alpha is zero from index n on, so row and
+        // column n of the copy add nothing to the rows kept)
+        if (l < 16) {
 #pragma unroll
-        for (int J = 0; J < TD; ++J) xs[16 * J + l] = al[J];
-    }
-    wave_sync();
-    float ec[TD];
-#pragma unroll
-    for (int K = 0; K < TD; ++K) {
-        double cp = 0.0;
-#pragma unroll
-        for (int J = 0; J <= K; ++J) {
-            const f32x4 aJ = *reinterpret_cast<const f32x4 *>(xs + 16 * J + 4 * q);
-            const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(K, J) * 256 + 4 * l);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) cp += (double)kt[r] * (double)aJ[r];
+            for (int J = 0; J < TD; ++J) xs[16 * J + l] = al[J];
         }
-        double rp[4] = {0.0, 0.0, 0.0, 0.0};
+        wave_sync();
+        float ec[TD];
 #pragma unroll
-        for (int I = K + 1; I < TD; ++I) {
-            const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(I, K) * 256 + 4 * l);
+        for (int K = 0; K < TD; ++K) {
+            double cp = 0.0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) rp[r] += (double)kt[r] * (double)al[I];
+            for (int J = 0; J <= K; ++J) {
+                const f32x4 aJ = *reinterpret_cast<const f32x4 *>(xs + 16 * J + 4 * q);
+                const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(K, J) * 256 + 4 * l);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cp += (double)kt[r] * (double)aJ[r];
+            }
+            double rp[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int I = K + 1; I < TD; ++I) {
+                const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(I, K) * 256 + 4 * l);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) rp[r] += (double)kt[r] * (double)al[I];
+            }
+            const double rsum = (K + 1 < TD) ? rows16_scatter_d(rp) : 0.0;
+            const double y = q4_sum_d(cp) + __shfl(rsum, ((c >> 2) << 4) | c, WAVE);
+            ec[K] = (16 * K + c < n) ? (float)((double)rc[K] - y) : 0.f;
         }
-        const double rsum = (K + 1 < TD) ? rows16_scatter_d(rp) : 0.0;
-        const double y = q4_sum_d(cp) + __shfl(rsum, ((c >> 2) << 4) | c, WAVE);
-        ec[K] = (16 * K + c < n) ? (float)((double)rc[K] - y) : 0.f;
+        correct<TD>(acc, n, ec, al);
+        wave_sync();  // every lane has read alpha from xs before x is written there
     }
-    correct<TD>(acc, n, ec, al);
-    wave_sync();  // every lane has read alpha from xs before x is written there
     // x = Y^T alpha: lane (q, c) multiplies segment 4s + q of row 16I + c by
-    // alpha[16I + c]; the sum over c lands on lanes c (even) for factor
-    // 8 (4s + q) + c / 2.
+    // alpha[16I + c]; after the scattered sum over c, lane c holds factor
+    // 8 (4s + q) + c / 2 (16-bit) or 4 (4s + q) + c % 4 (FP32).
     for (int s = 0; s < ks; ++s) {
         u32x4 F[TD];
         dual_operands<TD, T, E>(a, id, s, nseg, F);
-        float p[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int I = 0; I < TD; ++I)
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                p[2 * w] = fmaf(from16<E>((uint16_t)(F[I][w] & 0xffffu)), al[I], p[2 * w]);
-                p[2 * w + 1] = fmaf(from16<E>((uint16_t)(F[I][w] >> 16)), al[I], p[2 * w + 1]);
-            }
-        const float z = rows16_scatter8(p);
         const int seg = 4 * s + q;
-        if ((c & 1) == 0 && seg < nseg) xs[8 * seg + (c >> 1)] = z;
+        if constexpr (E == PREC_FP32) {
+            float p[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int I = 0; I < TD; ++I)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) p[j] = fmaf(__uint_as_float(F[I][j]), al[I], p[j]);
+            const float z = rows16_scatter4(p);
+            if (c < 4 && seg < nseg) xs[4 * seg + c] = z;
+        } else {
+            float p[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int I = 0; I < TD; ++I)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    p[2 * w] = fmaf(from16<E>((uint16_t)(F[I][w] & 0xffffu)), al[I], p[2 * w]);
+                    p[2 * w + 1] = fmaf(from16<E>((uint16_t)(F[I][w] >> 16)), al[I], p[2 * w + 1]);
+                }
+            const float z = rows16_scatter8(p);
+            if ((c & 1) == 0 && seg < nseg) xs[8 * seg + (c >> 1)] = z;
+        }
     }
     wave_sync();
 #pragma unroll
@@ -1138,14 +1185,14 @@ __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n,
 }
 
 // The dual system of the fewest tiles that hold n + 1 rows.
-template <int TD, int TDMAX, int T, int E>
+template <int TD, int TDMAX, int T, int E, bool REFINE>
 __device__ __forceinline__ void dual_pick(const UpdateArgs &a, int64_t rs, int n, float reg, float (&xc)[T],
                                           float *xs, float *kst) {
     if constexpr (TD < TDMAX) {
-        if (n < 16 * TD) dual_row<TD, T, E>(a, rs, n, reg, xc, xs, kst);
-        else dual_pick<TD + 1, TDMAX, T, E>(a, rs, n, reg, xc, xs, kst);
+        if (n < 16 * TD) dual_row<TD, T, E, REFINE>(a, rs, n, reg, xc, xs, kst);
+        else dual_pick<TD + 1, TDMAX, T, E, REFINE>(a, rs, n, reg, xc, xs, kst);
     } else {
-        dual_row<TD, T, E>(a, rs, n, reg, xc, xs, kst);
+        dual_row<TD, T, E, REFINE>(a, rs, n, reg, xc, xs, kst);
     }
 }
 
@@ -1167,13 +1214,14 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
 #pragma unroll
             for (int J = 0; J < T; ++J) x[J] = 0.f;
             bool dual = false;
-            if constexpr (S == SOLVE_CHOL_WARM && G::DUAL_TD > 0) {
-                // Whole rows under 16 * DUAL_TD ratings: the dual system.
+            if constexpr ((S == SOLVE_CHOL || S == SOLVE_CHOL_WARM) && G::DUAL_TD > 0) {
+                // Whole rows under 16 * DUAL_TD ratings: the dual system,
+                // refined once where the solve carries a refinement step.
                 if (n > 0 && n < 16 * G::DUAL_TD && w.w < 0 && !a.Lsave) {
                     const float reg = (a.lambda_rows ? a.lambda_rows[row] : a.lambda) * (float)n;
                     float *xsd = reinterpret_cast<float *>(lds) + 64;
                     float *kst = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);
-                    dual_pick<1, G::DUAL_TD, T, E>(a, w.y, n, reg, x, xsd, kst);
+                    dual_pick<1, G::DUAL_TD, T, E, S == SOLVE_CHOL_WARM>(a, w.y, n, reg, x, xsd, kst);
                     dual = true;
                     bool finite = true;
 #pragma unroll
