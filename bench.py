@@ -213,10 +213,18 @@ def main() -> int:
     half, ms, rows_nnz = ("items", im, d.nnz) if im >= um else ("users", um, d.nnz)
     gram_flops = float(d.nnz) * f * (f + 1)           # lower triangle incl. diagonal, 2 flop per MAC
     achieved = gram_flops / (ms * 1e-3) / 1e12
+    # The launch also factors one (f+1)x(f+1) system per row in FP32: (f+1)^3/3 MACs. Its time
+    # bound at the FP32 matrix peak, added to the Gram's at the FP16 peak, gives the launch's
+    # compute roofline time; frac_time compares it with the measured time.
+    rows = d.m if half == "users" else d.n
+    solve_flops = float(rows) * 2.0 * (f + 1) ** 3 / 3.0
+    roof_ms = (gram_flops / (PEAK_FP16_TFLOPS * 1e12) + solve_flops / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP16_TFLOPS, "traffic": parse_traffic(f"update_{half}"),
                 "kernel": f"update_kernel (16-bit Gram + Cholesky), {half} half-sweep",
-                "kernel_ms": ms, "flops_per_launch": gram_flops}
+                "kernel_ms": ms, "flops_per_launch": gram_flops,
+                "solve_fp32_flops_per_launch": solve_flops, "roofline_ms_gram_fp16_plus_solve_fp32": roof_ms,
+                "frac_time": roof_ms / ms}
 
     line = {
         "metric": METRIC, "value": a.steps / dt_m, "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,

@@ -438,3 +438,38 @@ def test_dual_system_for_light_rows(toy, f, prec, oprec):
     light = np.diff(toy.csc_ptr) < 48
     print("users", eu.max(), "light items", ev[light].max(), "other items", ev[~light].max())
     assert eu.max() <= 1e-6 and ev[light].max() <= 1e-6 and ev.max() <= 1e-5
+
+
+@pytest.mark.parametrize("f", [64, 100])
+@pytest.mark.parametrize("prec,refine,warm,oprec,tol", [(E.Prec.FP32, 0, False, O.PREC_EXACT, 1e-4),
+                                                        (E.Prec.FP16, 0, False, O.PREC_FP16, 1e-4),
+                                                        (E.Prec.FP16, 1, True, O.PREC_FP16, 1e-5)])
+def test_dual_system_tile_boundaries(f, prec, refine, warm, oprec, tol):
+    """Rows whose rating count puts the augmented row n at every position
+    the dual system distinguishes: first and last row of a tile (n = 15, 16,
+    31, 32, 47), the one-rating row, the empty row, and rows just past the
+    dual limit (48, 49, 63) or far past it, which take the primal system.
+    One users half-sweep from random factors against the FP64 oracle
+    (relative to max(1, |x|)); plain solves 1e-4, warm-started 1e-5."""
+    rng = np.random.default_rng(17)
+    counts = np.repeat([0, 1, 2, 14, 15, 16, 17, 30, 31, 32, 33, 46, 47, 48, 49, 63, 64, 100], 3)
+    m, n_items = len(counts), 400
+    cols = np.concatenate([np.sort(rng.choice(n_items, c, replace=False)) for c in counts]).astype(np.int32)
+    ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    vals = rng.integers(1, 6, cols.size).astype(np.float32)
+    rows = np.repeat(np.arange(m), counts)
+    d = ca.from_arrays(m, n_items, ptr, cols, vals, rows[:5], cols[:5], vals[:5])
+    U0 = (0.3 * rng.standard_normal((m, f))).astype(np.float32)
+    V0 = (0.3 * rng.standard_normal((n_items, f))).astype(np.float32)
+    with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=refine, refine_warm=warm), d) as e:
+        e.set_factors(0, U0)
+        e.set_factors(1, V0)
+        e.update(ca.ALS_SIDE_USERS)
+        Ug = e.get_factors(0)
+        assert e.info().failures == 0
+    Un, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec), d.csr_ptr, d.csr_idx, d.csr_val, V0,
+                     U0.astype(np.float64))
+    err = np.abs(Ug - Un).max(axis=1) / np.maximum(1.0, np.abs(Un).max(axis=1))
+    print({int(c): float(e) for c, e in zip(counts[::3], err.reshape(-1, 3).max(axis=1))})
+    assert not Ug[counts == 0].any()
+    assert err.max() <= tol
