@@ -86,9 +86,10 @@ def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True):
 
 
 def cpu_baseline(d, f, lam, budget_s=20.0):
-    """FP64 oracle (Cholesky) on uniformly sampled rows of both halves, scaled
-    to a full sweep; timed with every host thread, then with one (a smaller
-    sample). Returns the dict for the JSON line."""
+    """FP64 oracle (Cholesky) on rows of both halves drawn uniformly with
+    replacement, in doubling batches until the time budget or the row count
+    is reached, scaled to a full sweep; timed with every host thread, then
+    with one (a smaller sample). Returns the dict for the JSON line."""
     import numpy as np
     from oracle import als_oracle as O
     rng = np.random.default_rng(0)
@@ -121,9 +122,9 @@ def cpu_baseline(d, f, lam, budget_s=20.0):
     O.lib.orc_set_num_threads(cores)
     return {"value": 1.0 / s_all, "unit": "sweeps/s", "cores": cores, "kind": "port",
             "value_1core": 1.0 / s_one,
-            "sample": (f"FP64 oracle Cholesky update of {pr['users'][1]} uniformly sampled users and "
-                       f"{pr['items'][1]} sampled items ({cores} threads; {pr1['users'][1]} + {pr1['items'][1]} "
-                       f"at 1 thread), per-row time scaled to all {d.m} users + {d.n} items")}
+            "sample": (f"FP64 oracle Cholesky update of {pr['users'][1]} user rows and {pr['items'][1]} item "
+                       f"rows drawn uniformly with replacement ({cores} threads; {pr1['users'][1]} + "
+                       f"{pr1['items'][1]} at 1 thread), per-row time scaled to all {d.m} users + {d.n} items")}
 
 
 def copy_calibration(torch, nbytes: int, steps: int) -> int:
