@@ -303,12 +303,6 @@ __device__ __forceinline__ double dpp_mov_d(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Lane N of this lane's row (DPP row_newbcast).
-template <int N>
-__device__ __forceinline__ float row_bcast(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + N, 0xf, 0xf, true));
-}
-
 // Sum over the 16 lanes of a row. Quad sums, then mirror and rotate-by-8
 // pairings: every lane adds the same operands in the same order, so all 16
 // lanes hold identical bits.
@@ -442,8 +436,9 @@ __device__ __forceinline__ void diag_steps(f32x4 &d, f32x4 &w, int jend) {
             // the permlane swaps see a full wave and need no branch.
             const float rb = rows_bcast<JG>(d[JR]);
             const float coef = (c > J) ? -rb * (rs * rs) : 0.f;  // -D[J][c] / P[J]
-            // d[r] += row_bcast<J>(d[r]) * coef   (-= D[4q+r][J] D[J][c] / P[J])
-            // w[r] += row_bcast<J>(w[r]) * coef   (-= L~[c][J] W~[J][4q+r])
+            // d[r] += bcastJ(d[r]) * coef   (-= D[4q+r][J] D[J][c] / P[J])
+            // w[r] += bcastJ(w[r]) * coef   (-= L~[c][J] W~[J][4q+r])
+            // bcastJ(v): v of lane J of this lane's row (DPP row_newbcast:J)
             // with the broadcast folded into the FMA as its DPP source operand:
             // one instruction per element instead of a DPP move plus an FMA.
             // s_nop 1 gives the two wait states a DPP read needs after a VALU
