@@ -104,6 +104,8 @@ SIGNATURES = {
     "als_group_last_error": (C.c_char_p, [P]),
     "als_group_update": (C.c_int, [P, C.c_int]),
     "als_group_sweep": (C.c_int, [P]),
+    "als_cg_update_x": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
+    "als_cg_update_x_fp16": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
     # als_data.h
     "als_synth_generate": (C.c_int, [C.POINTER(SynthSpec), C.POINTER(P)]),
     "als_dataset_from_csr": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_int64, P, P, P, C.POINTER(P)]),

@@ -904,7 +904,9 @@ __device__ __forceinline__ void cg_regs(const f32x4 (&A)[NT], int f, const float
             r[J] -= alpha * ap[J];
         }
         const float rsnew = colform_dot(r, r, T);
-        if (rsnew < tol) break;
+        // r.r == 0 (converged, or its square underflowed): the next beta
+        // would be 0 / 0, so stop with the current x
+        if (rsnew < tol || rsnew == 0.f) break;
         const float beta = rsnew / rsold;
         rsold = rsnew;
 #pragma unroll

@@ -949,6 +949,40 @@ extern "C" int als_group_sweep(als_group *g) {
     return rc ? rc : als_group_update(g, ALS_SIDE_ITEMS);
 }
 
+// Batched CG on caller device buffers, synchronous like the reference's
+// updateXWithCGHost (cumf_als_cg/cg.h:30-32, cg.cu:683-688): the kernel is
+// enqueued on hip_stream (null: the default stream) of the current device,
+// and the call returns once that stream has finished it.
+static int cg_update_x(const float *A, const uint16_t *A16, float *x, const float *b, int batch, int f, int cg_iters,
+                       float tol, void *hip_stream) {
+    if (batch < 0 || f < 1 || f > 143 || cg_iters < 0 || !(tol >= 0.f)) return ALS_E_INVALID;
+    if (batch == 0) return ALS_OK;
+    if ((!A && !A16) || !x || !b) return ALS_E_INVALID;
+    CgBatchArgs a{};
+    a.A = A;
+    a.A16 = A16;
+    a.x = x;
+    a.b = b;
+    a.batch = batch;
+    a.f = f;
+    a.iters = cg_iters;
+    a.tol = tol;
+    hipStream_t s = reinterpret_cast<hipStream_t>(hip_stream);
+    if (launch_cg_batch((f + 16) / 16, a, s) != hipSuccess) return ALS_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return ALS_E_HIP;
+    return ALS_OK;
+}
+
+extern "C" int als_cg_update_x(const float *A, float *x, const float *b, int batch, int f, int cg_iters, float tol,
+                               void *hip_stream) {
+    return cg_update_x(A, nullptr, x, b, batch, f, cg_iters, tol, hip_stream);
+}
+
+extern "C" int als_cg_update_x_fp16(const uint16_t *A, float *x, const float *b, int batch, int f, int cg_iters,
+                                    float tol, void *hip_stream) {
+    return cg_update_x(nullptr, A, x, b, batch, f, cg_iters, tol, hip_stream);
+}
+
 extern "C" int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_work, int grid,
                                     int gram_prec, als_schedule_stats *out, int *units_out, int64_t max_units) {
     if (!ptr || !out || rows < 0 || grid <= 0 || gram_prec < ALS_PREC_FP32 || gram_prec > ALS_PREC_FP16)

@@ -204,6 +204,60 @@ __global__ __launch_bounds__(WAVE) void ybar_kernel(YbarArgs a) {
     }
 }
 
+// One wave per system: the lower 16x16 tiles of A in the register layout
+// of the update kernel (diagonal tiles A[16I+4q+r][16I+c], off-diagonal
+// tiles row by lane A[16I+c][16J+4q+r]), zero outside the f x f block,
+// then the same register CG as the fused update (cumf recurrence: warm
+// start, cg_iters cap, exit once r.r < tol after an update, pAp == 0 guard).
+template <int T>
+__global__ __launch_bounds__(WAVE) void cg_batch_kernel(CgBatchArgs a) {
+    constexpr int NT = T * (T + 1) / 2;
+    const int l = lane(), q = l >> 4, c = l & 15;
+    const int f = a.f;
+    for (int sys = blockIdx.x; sys < a.batch; sys += gridDim.x) {
+        const int64_t base = (int64_t)sys * f * f;
+        auto at = [&](int i, int j) -> float {
+            if (i >= f || j >= f) return 0.f;
+            const int64_t k = base + (int64_t)i * f + j;
+            if (a.A16) return (float)__builtin_bit_cast(_Float16, a.A16[k]);
+            return a.A[k];
+        };
+        f32x4 acc[NT];
+#pragma unroll
+        for (int I = 0; I < T; ++I)
+#pragma unroll
+            for (int J = 0; J <= I; ++J)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[tile_id(I, J)][r] = (I == J) ? at(16 * I + 4 * q + r, 16 * I + c)
+                                                     : at(16 * I + c, 16 * J + 4 * q + r);
+        float bc[T], xc[T];
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+            const int i = 16 * J + c;
+            bc[J] = i < f ? a.b[(int64_t)sys * f + i] : 0.f;
+            xc[J] = i < f ? a.x[(int64_t)sys * f + i] : 0.f;
+        }
+        cg_regs<T>(acc, f, bc, xc, a.iters, a.tol);
+        if (q == 0) {
+#pragma unroll
+            for (int J = 0; J < T; ++J)
+                if (16 * J + c < f) a.x[(int64_t)sys * f + 16 * J + c] = xc[J];
+        }
+    }
+}
+
+hipError_t launch_cg_batch(int T, const CgBatchArgs &a, hipStream_t s) {
+    const int grid = std::max(1, std::min(a.batch, 65536));
+    switch (T) {
+#define ALS_CGB(TT) case TT: cg_batch_kernel<TT><<<grid, WAVE, 0, s>>>(a); break;
+    ALS_CGB(1) ALS_CGB(2) ALS_CGB(3) ALS_CGB(4) ALS_CGB(5) ALS_CGB(6) ALS_CGB(7) ALS_CGB(8) ALS_CGB(9)
+#undef ALS_CGB
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s) {
     const int grid = std::max(1, std::min(a.rows, 65536));
     const size_t lds = 0;

@@ -66,6 +66,20 @@ struct AdjointArgs {
     int f;
 };
 
+// Batched CG on caller-provided dense systems (the reference's
+// updateXWithCGHost boundary): A batch x f x f (symmetric, either order),
+// x and b batch x f, all densely packed.
+struct CgBatchArgs {
+    const float *A;             // FP32 systems, or nullptr when A16 is set
+    const uint16_t *A16;        // FP16 systems
+    float *x;                   // warm start in, solution out
+    const float *b;
+    int batch;
+    int f;
+    int iters;
+    float tol;
+};
+
 struct YbarArgs {
     const int *cptr;            // CSC of the same matrix (columns = fixed-factor rows)
     const int *cidx;
@@ -93,6 +107,7 @@ const void *update_kernel_ptr(int T, int prec, int solve);
 hipError_t launch_update_group(int prec, int solve, const GroupArgs &g, int grid, int lds_bytes, hipStream_t s);
 const void *update_group_kernel_ptr(int prec, int solve);
 hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s);
+hipError_t launch_cg_batch(int T, const CgBatchArgs &a, hipStream_t s);
 hipError_t launch_ybar(int T, const YbarArgs &a, hipStream_t s);
 hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, hipStream_t s);
 hipError_t launch_to16(int prec, const float *src, uint16_t *dst, int64_t count, hipStream_t s);

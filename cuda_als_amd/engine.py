@@ -168,6 +168,28 @@ class Engine:
                                         C.c_void_p(d_xbar), C.c_void_p(d_lbar), C.c_void_p(d_ybar or 0)))
 
 
+def cg_update_x(A, x, b, cg_iters: int = 6, tol: float = 1e-4) -> None:
+    """Batched CG on device tensors, the reference's updateXWithCGHost
+    (cumf_als_cg/cg.h:30; FP16 A: updateXWithCGHost_tt_fp16, cg.h:32).
+    A: (batch, f, f) float32 or float16, symmetric positive definite; x:
+    (batch, f) float32, warm start in and solution out; b: (batch, f)
+    float32; all contiguous on the current HIP device. Runs on torch's
+    current stream and returns when it has finished, like the reference."""
+    import torch
+    if A.dim() != 3 or A.shape[1] != A.shape[2] or x.shape != A.shape[:2] or b.shape != A.shape[:2]:
+        raise ValueError("A must be (batch, f, f), x and b (batch, f)")
+    if x.dtype != torch.float32 or b.dtype != torch.float32 or A.dtype not in (torch.float32, torch.float16):
+        raise ValueError("x, b float32; A float32 or float16")
+    if not (A.is_cuda and x.is_cuda and b.is_cuda) or not (A.is_contiguous() and x.is_contiguous()
+                                                          and b.is_contiguous()):
+        raise ValueError("A, x, b must be contiguous device tensors")
+    fn = lib.als_cg_update_x if A.dtype == torch.float32 else lib.als_cg_update_x_fp16
+    rc = fn(C.c_void_p(A.data_ptr()), C.c_void_p(x.data_ptr()), C.c_void_p(b.data_ptr()), int(A.shape[0]),
+            int(A.shape[1]), int(cg_iters), float(tol), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    if rc != 0:
+        raise AlsError(rc, "als_cg_update_x failed (f in 1..143, cg_iters >= 0, tol >= 0)")
+
+
 class Group:
     """Several engines (models of different f) over one rating matrix,
     updated by one kernel launch per half-sweep (als_group_*). The engines

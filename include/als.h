@@ -29,6 +29,10 @@
  *                              als_model.cu:1722-1764
  *   als_get_factors            h_UT / h_VT after train(), als_model.cu:2283-2284
  *   als_save_libmf             als_model::save_libmf, als_model.cu:2294-2312
+ *   als_cg_update_x / als_cg_update_x_fp16
+ *                              updateXWithCGHost / updateXWithCGHost_tt_fp16,
+ *                              cumf_als_cg/cg.h:30-32 (kernels cg.cu:37-232,
+ *                              236-430; host side cg.cu:642-688)
  *   als_forward_dev / als_adjoint_dev
  *                              no reference counterpart: differentiable
  *                              half-sweep (adjoint of the batched solve)
@@ -129,6 +133,26 @@ int als_rmse(als_engine *e, int which, double *out);
 int als_synchronize(als_engine *e);
 int als_get_info(als_engine *e, als_info *out);
 int als_save_libmf(als_engine *e, const char *path);
+
+/* Batched CG on caller device buffers: the reference's updateXWithCGHost
+ * (cumf_als_cg/cg.h:30) and updateXWithCGHost_tt_fp16 (cg.h:32), whose
+ * cgIter argument becomes cg_iters. A is batch x f x f (symmetric positive
+ * definite; row- or column-major alike), x (warm start in, solution out)
+ * and b are batch x f, all densely packed device pointers of the current
+ * device. Recurrence of the reference kernel (cg.cu:37-232): r = b - A x,
+ * p = r; per iteration alpha = r.r / p.A p, x += alpha p, r -= alpha A p,
+ * exit once r.r < tol, else p = r + beta p; a zero p.A p, or a zero r.r
+ * after an update, ends the loop (the reference divides 0 / 0 there).
+ * tol takes the place of the reference's compile-time CG_ERROR (1e-4,
+ * cg.cu:32).
+ * The _fp16 form takes A stored as IEEE half (the reference's own kernel
+ * reinterprets FP32 bits as half, als_model.cu:2003; not reproduced).
+ * f in 1..143. Enqueued on hip_stream (null: the default stream) and
+ * synchronous, like the reference: returns once the stream has run it. */
+int als_cg_update_x(const float *A, float *x, const float *b, int batch, int f, int cg_iters, float tol,
+                    void *hip_stream);
+int als_cg_update_x_fp16(const uint16_t *A, float *x, const float *b, int batch, int f, int cg_iters, float tol,
+                         void *hip_stream);
 
 /* Host-only (no device needed): the work schedule the engine builds for one
  * side from its CSR row pointers: whole rows plus parts of rows longer than

@@ -149,7 +149,8 @@ int orc_lu_nopiv_solve(int f, double *A, double *b) {
  *   repeat cgIter times: Ap = A p; alpha = rsold / p.Ap; x += alpha p;
  *   r -= alpha Ap; rsnew = r.r; if rsnew < CG_ERROR break (:196);
  *   p = r + (rsnew/rsold) p; rsold = rsnew.
- * Added guard: p.Ap == 0 stops the loop (the reference divides 0/0). */
+ * Added guards: p.Ap == 0 stops the loop, and so does r.r == 0 after an
+ * update (the reference divides 0/0 in both cases). */
 int orc_cg_solve(int f, const double *A, const double *b, double *x, int iters, double tol) {
     double *r = (double *)malloc(sizeof(double) * (size_t)f * 3);
     double *p = r + f, *ap = r + 2 * f;
@@ -177,7 +178,7 @@ int orc_cg_solve(int f, const double *A, const double *b, double *x, int iters, 
             r[i] -= alpha * ap[i];
             rsnew += r[i] * r[i];
         }
-        if (rsnew < tol) break;
+        if (rsnew < tol || rsnew == 0.0) break; /* r.r == 0: beta would be 0/0 */
         double beta = rsnew / rsold;
         rsold = rsnew;
         for (int i = 0; i < f; i++) p[i] = r[i] + beta * p[i];
