@@ -99,11 +99,17 @@ template <int T, int E> struct Geo {
     static constexpr int NT = T * (T + 1) / 2;  // lower-triangle 16x16 tiles
     static constexpr int FP = 16 * T;           // padded augmented dimension
     static constexpr int YT_H = 40;             // 16-bit stage row: 32 ratings + 8 pad (80 B)
+    // Factor row j of the 16-bit stage sits at row j + j / 8: the gather
+    // writes rows 8s + 2v (s = 0..12 in one instruction) and the MFMA reads
+    // rows 16I + c, and the skew spreads both over all eight 16-byte bank
+    // groups of a 128-byte LDS line (without it the writes of one
+    // instruction share four).
+    __host__ __device__ static constexpr int yt_row(int j) { return j + (j >> 3); }
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
     // Rows with fewer than 16 * DUAL_TD ratings take the dual system
     // (T >= 5), whose FP32 tile copy lives in the stage.
     static constexpr int DUAL_TD = T >= 5 ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
-    static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : FP * YT_H * 2;
+    static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : yt_row(FP) * YT_H * 2;
     static constexpr int DUAL_STAGE_BYTES = DUAL_TD * (DUAL_TD + 1) / 2 * 1024;
     static constexpr int STAGE_BYTES = GRAM_STAGE_BYTES > DUAL_STAGE_BYTES ? GRAM_STAGE_BYTES : DUAL_STAGE_BYTES;
     // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
@@ -130,7 +136,7 @@ __device__ __forceinline__ void stage_clear(int f, char *stage) {
         const int nseg = (f + 7) >> 3;  // 16-byte segments carrying data
         for (int e = l; e < (G::FP - 8 * nseg) * CHUNK; e += WAVE) {
             int j = 8 * nseg + e / CHUNK, k = e % CHUNK;
-            Yt[j * G::YT_H + k] = 0;
+            Yt[G::yt_row(j) * G::YT_H + k] = 0;
         }
     }
 }
@@ -196,12 +202,12 @@ __device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int
                     lo[w] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);  // low halves of x0, x1
                     hi[w] = __builtin_amdgcn_perm(x1, x0, 0x07060302u);  // high halves
                 }
-                *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v) * G::YT_H + 8 * g) = lo;
-                *reinterpret_cast<u32x4 *>(Yt + (8 * s + 2 * v + 1) * G::YT_H + 8 * g) = hi;
+                *reinterpret_cast<u32x4 *>(Yt + G::yt_row(8 * s + 2 * v) * G::YT_H + 8 * g) = lo;
+                *reinterpret_cast<u32x4 *>(Yt + G::yt_row(8 * s + 2 * v + 1) * G::YT_H + 8 * g) = hi;
             }
         }
         wave_sync();
-        if (l < CHUNK) Yt[f * G::YT_H + l] = to16<E>(rbuf[l]);
+        if (l < CHUNK) Yt[G::yt_row(f) * G::YT_H + l] = to16<E>(rbuf[l]);
     }
     wave_sync();
 }
@@ -242,7 +248,7 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
             u32x4 F[T];
 #pragma unroll
             for (int I = 0; I < T; ++I)
-                F[I] = *reinterpret_cast<const u32x4 *>(Yt + (16 * I + (l & 15)) * G::YT_H + 8 * (l >> 4));
+                F[I] = *reinterpret_cast<const u32x4 *>(Yt + G::yt_row(16 * I + (l & 15)) * G::YT_H + 8 * (l >> 4));
 #pragma unroll
             for (int I = 0; I < T; ++I)
 #pragma unroll
@@ -736,7 +742,7 @@ __device__ __forceinline__ float stage_at(const char *stage, int k, int j) {
     if constexpr (E == PREC_FP32)
         return reinterpret_cast<const float *>(stage)[k * G::YR + j];
     else
-        return from16<E>(reinterpret_cast<const uint16_t *>(stage)[j * G::YT_H + k]);
+        return from16<E>(reinterpret_cast<const uint16_t *>(stage)[G::yt_row(j) * G::YT_H + k]);
 }
 
 // r_j += sum_k y_k[j] e_k over the staged chunk, for factor j.
@@ -749,7 +755,7 @@ __device__ __forceinline__ double stage_row_dot(const char *stage, int j, const 
 #pragma unroll 8
         for (int k = 0; k < CHUNK; ++k) acc += (double)Ys[k * G::YR + j] * eb[k];
     } else {
-        const u32x4 *p = reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint16_t *>(stage) + j * G::YT_H);
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint16_t *>(stage) + G::yt_row(j) * G::YT_H);
 #pragma unroll
         for (int v = 0; v < CHUNK / 8; ++v) {
             const u32x4 w = p[v];
