@@ -1200,7 +1200,10 @@ __device__ __forceinline__ void dual_pick(const UpdateArgs &a, int64_t rs, int n
 }
 
 // Every work unit of queue batch b of one model.
-template <int T, int E, int S>
+// DUAL_INLINE: light rows take the dual system inside this batch loop (the
+// multi-model kernel); the update kernels leave them to dual_kernel, so the
+// dual code does not weigh on their register allocation.
+template <int T, int E, int S, bool DUAL_INLINE = false>
 __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds) {
     using G = Geo<T, E>;
     const int l = lane();
@@ -1217,7 +1220,7 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
 #pragma unroll
             for (int J = 0; J < T; ++J) x[J] = 0.f;
             bool dual = false;
-            if constexpr ((S == SOLVE_CHOL || S == SOLVE_CHOL_WARM) && G::DUAL_TD > 0) {
+            if constexpr (DUAL_INLINE && (S == SOLVE_CHOL || S == SOLVE_CHOL_WARM) && G::DUAL_TD > 0) {
                 // Whole rows under 16 * DUAL_TD ratings: the dual system,
                 // refined once where the solve carries a refinement step.
                 if (n > 0 && n < 16 * G::DUAL_TD && w.w < 0 && !a.Lsave) {
@@ -1358,6 +1361,49 @@ __device__ __forceinline__ void update_body(const UpdateArgs &a) {
 template <int T, int E, int S>
 __global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
     update_body<T, E, S>(a);
+}
+
+// Light rows through the dual system: whole rows with fewer than
+// 16 * DUAL_TD ratings (and empty rows, x = 0), the units of their own
+// schedule, launched after the update kernel of the same half-sweep. With
+// only 1-3 tiles live it needs far fewer registers than the update kernel
+// and runs at a higher occupancy.
+template <int T, int E, int S>
+__global__ __launch_bounds__(WAVE) void dual_kernel(UpdateArgs a) {
+    using G = Geo<T, E>;
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int l = lane();
+    float *xs = reinterpret_cast<float *>(lds) + 64;
+    float *kst = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);
+    for (;;) {
+        const int b = pop_batch(a.queue);
+        if (b >= a.nbatches) break;
+        const int u0 = a.batch_begin[b], u1 = a.batch_begin[b + 1];
+        for (int u = u0; u < u1; ++u) {
+            const int row = a.units[u].x;
+            const int n = a.ptr[row + 1] - a.ptr[row];
+            float x[T];
+#pragma unroll
+            for (int J = 0; J < T; ++J) x[J] = 0.f;
+            if (n > 0) {
+                const float reg = (a.lambda_rows ? a.lambda_rows[row] : a.lambda) * (float)n;
+                dual_pick<1, G::DUAL_TD, T, E, S == SOLVE_CHOL_WARM>(a, a.ptr[row], n, reg, x, xs, kst);
+                bool finite = true;
+#pragma unroll
+                for (int J = 0; J < T; ++J) finite = finite && isfinite(x[J]);
+                if (__any(!finite)) {
+                    if (l == 0) atomicAdd(a.fail_count, 1);
+#pragma unroll
+                    for (int J = 0; J < T; ++J) x[J] = 0.f;
+                }
+            }
+            store_colform<T>(a.X + (int64_t)row * G::FP, x);
+            if constexpr (E != PREC_FP32) {
+                if (a.Xs) store_colform16<T, E>(a.Xs + (int64_t)row * G::FP, x);
+            }
+        }
+    }
+    leave_queue(a.queue);
 }
 
 // The refinement variant holds the factor, the solution and the FP64

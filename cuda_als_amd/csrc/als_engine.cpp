@@ -37,6 +37,11 @@ struct Side {
     int grid = 0;
     int restage_max = 0;           // refinement: restage rows up to this long (else tile copy)
     float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
+    // Light rows (fewer than dual_max ratings, and empty rows) taken by the
+    // dual kernel: their own schedule, whole rows only.
+    int4 *dunits = nullptr;
+    int *dbatch = nullptr;
+    int dnbatches = 0, dnunits = 0;
 };
 
 }  // namespace
@@ -44,6 +49,7 @@ struct Side {
 struct als_engine {
     als_config cfg{};
     int T = 0, ld = 0, prec = PREC_FP32, solve = SOLVE_CHOL, lds = 0;
+    int dual_max = 0, dual_lds = 0, dual_grid = 0;  // dual kernel: rows below dual_max ratings (0: none)
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     std::string err;
@@ -120,7 +126,15 @@ int check_csr(als_engine *e, int64_t rows, int64_t cols, int64_t nnz, const int 
 
 // (Re)build one side's work schedule and upload it; replaces any previous one.
 int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work) {
-    Schedule sc = build_schedule(rows, ptr, (int)split_len, (int)batch_work, s.grid, e->prec);
+    // With a dual kernel, the light rows get a schedule of their own.
+    std::vector<int> heavy, light;
+    if (e->dual_max > 0) {
+        for (int64_t r = 0; r < rows; r++) (ptr[r + 1] - ptr[r] < e->dual_max ? light : heavy).push_back((int)r);
+    }
+    const bool dual = e->dual_max > 0;
+    Schedule sc = build_schedule(rows, ptr, (int)split_len, (int)batch_work, s.grid, e->prec, dual ? &heavy : nullptr);
+    Schedule sd;
+    if (dual && !light.empty()) sd = build_schedule(rows, ptr, (int64_t)1 << 30, 0, e->dual_grid, e->prec, &light);
     static_assert(sizeof(Unit) == sizeof(int4) && sizeof(Split) == sizeof(int4), "unit layout");
     HIPCHK(e, hipStreamSynchronize(e->stream));
     dev_free(e, s.units);
@@ -128,10 +142,25 @@ int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr, int64_t spl
     dev_free(e, s.splits);
     dev_free(e, s.counters);
     dev_free(e, s.partial);
+    dev_free(e, s.dunits);
+    dev_free(e, s.dbatch);
     s.units = s.splits = nullptr;
     s.batch = s.counters = nullptr;
     s.partial = nullptr;
+    s.dunits = nullptr;
+    s.dbatch = nullptr;
+    s.dnbatches = s.dnunits = 0;
     int rc;
+    if (!sd.units.empty()) {
+        if ((rc = dev_alloc(e, &s.dunits, sd.units.size())) || (rc = dev_alloc(e, &s.dbatch, sd.batch_begin.size())))
+            return rc;
+        HIPCHK(e, hipMemcpyAsync(s.dunits, sd.units.data(), sd.units.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                 e->stream));
+        HIPCHK(e, hipMemcpyAsync(s.dbatch, sd.batch_begin.data(), sd.batch_begin.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, e->stream));
+        s.dnbatches = (int)sd.batch_begin.size() - 1;
+        s.dnunits = (int)sd.units.size();
+    }
     if ((rc = dev_alloc(e, &s.units, sc.units.size())) || (rc = dev_alloc(e, &s.batch, sc.batch_begin.size())) ||
         (rc = dev_alloc(e, &s.splits, sc.splits.size())) || (rc = dev_alloc(e, &s.counters, sc.splits.size())) ||
         (rc = dev_alloc(e, &s.partial, (size_t)sc.nslots * (e->T * (e->T + 1) / 2) * 256)))
@@ -198,13 +227,32 @@ UpdateArgs side_args(als_engine *e, int sd, const float *Y, const uint16_t *Ys, 
     return a;
 }
 
-// Launch one half-sweep.
+// The same arguments over the dual schedule's units (whole rows only).
+UpdateArgs dual_args(const Side &s, UpdateArgs a) {
+    a.units = s.dunits;
+    a.batch_begin = s.dbatch;
+    a.nbatches = s.dnbatches;
+    a.splits = nullptr;
+    a.counters = nullptr;
+    a.partial = nullptr;
+    return a;
+}
+
+// Launch one half-sweep: the update kernel over its schedule, then the dual
+// kernel over the light rows. When the factorisation is saved for the
+// adjoint (Lsave), the light rows go through the update kernel too: the
+// adjoint needs every row's f x f factor.
 int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
                 const float *lambda_rows, float *Lsave) {
     const Side &s = e->side[sd];
-    if (s.rows == 0 || s.nbatches == 0) return ALS_OK;
+    if (s.rows == 0) return ALS_OK;
     const UpdateArgs a = side_args(e, sd, Y, Ys, X, Xs, lambda_rows, Lsave);
-    HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, e->stream));
+    if (s.nbatches > 0) HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, e->stream));
+    if (s.dnbatches > 0) {
+        const UpdateArgs d = dual_args(s, a);
+        if (Lsave) HIPCHK(e, launch_update(e->T, e->prec, e->solve, d, s.grid, e->lds, e->stream));
+        else HIPCHK(e, launch_dual(e->T, e->prec, e->solve, d, e->dual_grid, e->dual_lds, e->stream));
+    }
     return ALS_OK;
 }
 
@@ -350,6 +398,15 @@ int als_create(const als_config *cfg, als_engine **out) {
         // restaging a row moves 2 FP bytes per rating; a round trip of the
         // tile copy 2 NT KiB: equal at 32 (T + 1) ratings
         e->side[0].restage_max = e->side[1].restage_max = 32 * (e->T + 1);
+    }
+    const void *dk = dual_kernel_ptr(e->T, e->prec, e->solve);
+    if (!rc && dk) {
+        e->dual_lds = dual_lds_bytes(e->T, e->prec);
+        int dpc = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&dpc, dk, 64, e->dual_lds) != hipSuccess)
+            rc = set_err(e, ALS_E_HIP, "occupancy query failed (dual kernel)");
+        e->dual_grid = std::max(1, dpc) * e->num_cu;
+        e->dual_max = dual_max_ratings(e->T, e->prec);
     }
     if (!rc && e->solve == SOLVE_CHOL_REFINE)
         rc = dev_alloc(e, &e->scratch, (size_t)e->side[0].grid * (e->T * (e->T + 1) / 2) * 256);
@@ -595,8 +652,8 @@ int als_get_info(als_engine *e, als_info *o) {
     o->grid[1] = e->side[1].grid;
     o->lds_bytes = e->lds;
     for (int sd = 0; sd < 2; sd++) {
-        o->nbatches[sd] = e->side[sd].nbatches;
-        o->nunits[sd] = e->side[sd].nunits;
+        o->nbatches[sd] = e->side[sd].nbatches + e->side[sd].dnbatches;
+        o->nunits[sd] = e->side[sd].nunits + e->side[sd].dnunits;
         o->nsplit_rows[sd] = e->side[sd].nsplits;
         o->split_len[sd] = e->side[sd].split_len;
     }
@@ -924,21 +981,33 @@ extern "C" int als_group_update(als_group *g, int sd) {
     if (sd != ALS_SIDE_USERS && sd != ALS_SIDE_ITEMS) return gset_err(g, ALS_E_INVALID, "side must be users or items");
     als_engine *e0 = g->eng[0];
     GroupArgs ga{};
-    ga.nmodels = (int)g->eng.size();
     ga.queue = g->queue;
     ga.first[0] = 0;
-    for (int k = 0; k < ga.nmodels; k++) {
+    int ne = 0;  // entries: each model's schedule, then its light rows' (taken inline by the group kernel)
+    for (int k = 0; k < (int)g->eng.size(); k++) {
         als_engine *e = g->eng[k];
         drop_graph(e);
+        const Side &s = e->side[sd];
         UpdateArgs a = sd == ALS_SIDE_USERS ? side_args(e, 0, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr)
                                             : side_args(e, 1, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr);
         a.refine = e->solve == SOLVE_CHOL_REFINE ? e->cfg.refine_steps : 0;
         a.scratch = g->scratch.empty() ? nullptr : g->scratch[k];
-        ga.m[k] = a;
-        ga.T[k] = e->T;
-        ga.first[k + 1] = ga.first[k] + (e->side[sd].rows > 0 ? e->side[sd].nbatches : 0);
+        if (s.rows == 0) continue;
+        if (s.nbatches > 0) {
+            ga.m[ne] = a;
+            ga.T[ne] = e->T;
+            ga.first[ne + 1] = ga.first[ne] + s.nbatches;
+            ++ne;
+        }
+        if (s.dnbatches > 0) {
+            ga.m[ne] = dual_args(s, a);
+            ga.T[ne] = e->T;
+            ga.first[ne + 1] = ga.first[ne] + s.dnbatches;
+            ++ne;
+        }
     }
-    if (ga.first[ga.nmodels] == 0) return ALS_OK;
+    ga.nmodels = ne;
+    if (ne == 0) return ALS_OK;
     const hipError_t st = launch_update_group(g->prec, g->solve, ga, g->grid, g->lds, e0->stream);
     if (st != hipSuccess) return gset_err(g, ALS_E_HIP, std::string("group launch: ") + hipGetErrorString(st));
     return ALS_OK;

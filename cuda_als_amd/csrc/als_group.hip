@@ -23,14 +23,14 @@ __global__ __launch_bounds__(WAVE) void update_group_kernel(GroupArgs g) {
         while (k + 1 < g.nmodels && b >= g.first[k + 1]) ++k;
         const int lb = b - g.first[k];
         switch (g.T[k]) {
-        case 2: run_batch<2, E, S>(g.m[k], lb, lds); break;
-        case 3: run_batch<3, E, S>(g.m[k], lb, lds); break;
-        case 4: run_batch<4, E, S>(g.m[k], lb, lds); break;
-        case 5: run_batch<5, E, S>(g.m[k], lb, lds); break;
-        case 6: run_batch<6, E, S>(g.m[k], lb, lds); break;
-        case 7: run_batch<7, E, S>(g.m[k], lb, lds); break;
-        case 8: run_batch<8, E, S>(g.m[k], lb, lds); break;
-        case 9: run_batch<9, E, S>(g.m[k], lb, lds); break;
+        case 2: run_batch<2, E, S, true>(g.m[k], lb, lds); break;
+        case 3: run_batch<3, E, S, true>(g.m[k], lb, lds); break;
+        case 4: run_batch<4, E, S, true>(g.m[k], lb, lds); break;
+        case 5: run_batch<5, E, S, true>(g.m[k], lb, lds); break;
+        case 6: run_batch<6, E, S, true>(g.m[k], lb, lds); break;
+        case 7: run_batch<7, E, S, true>(g.m[k], lb, lds); break;
+        case 8: run_batch<8, E, S, true>(g.m[k], lb, lds); break;
+        case 9: run_batch<9, E, S, true>(g.m[k], lb, lds); break;
         default: break;
         }
     }

@@ -42,6 +42,50 @@ hipError_t launch_update(int T, int prec, int solve, const UpdateArgs &a, int gr
     }
 }
 
+const void *dual_kernel_ptr_p0(int T, int solve);
+const void *dual_kernel_ptr_p1(int T, int solve);
+const void *dual_kernel_ptr_p2(int T, int solve);
+int dual_lds_bytes_p0(int T);
+int dual_lds_bytes_p1(int T);
+int dual_lds_bytes_p2(int T);
+int dual_max_ratings_p0(int T);
+int dual_max_ratings_p1(int T);
+int dual_max_ratings_p2(int T);
+
+const void *dual_kernel_ptr(int T, int prec, int solve) {
+    switch (prec) {
+    case PREC_FP32: return dual_kernel_ptr_p0(T, solve);
+    case PREC_BF16: return dual_kernel_ptr_p1(T, solve);
+    case PREC_FP16: return dual_kernel_ptr_p2(T, solve);
+    default: return nullptr;
+    }
+}
+
+int dual_lds_bytes(int T, int prec) {
+    switch (prec) {
+    case PREC_FP32: return dual_lds_bytes_p0(T);
+    case PREC_BF16: return dual_lds_bytes_p1(T);
+    case PREC_FP16: return dual_lds_bytes_p2(T);
+    default: return 0;
+    }
+}
+
+int dual_max_ratings(int T, int prec) {
+    switch (prec) {
+    case PREC_FP32: return dual_max_ratings_p0(T);
+    case PREC_BF16: return dual_max_ratings_p1(T);
+    case PREC_FP16: return dual_max_ratings_p2(T);
+    default: return 0;
+    }
+}
+
+hipError_t launch_dual(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s) {
+    const void *k = dual_kernel_ptr(T, prec, solve);
+    if (!k) return hipErrorInvalidValue;
+    void *args[] = {const_cast<UpdateArgs *>(&a)};
+    return hipLaunchKernel(k, dim3(grid), dim3(WAVE), args, (size_t)lds, s);
+}
+
 const void *update_group_kernel_ptr_p0(int solve);
 const void *update_group_kernel_ptr_p1(int solve);
 const void *update_group_kernel_ptr_p2(int solve);

@@ -9,11 +9,17 @@ namespace als {
 
 int64_t row_solve_cost(int prec) { return prec == PREC_FP32 ? 96 : 1024; }
 
-Schedule build_schedule(int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work, int grid, int prec) {
+Schedule build_schedule(int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work, int grid, int prec,
+                        const std::vector<int> *subset) {
     Schedule s;
     const int64_t solve = row_solve_cost(prec);
+    const int64_t nrows = subset ? (int64_t)subset->size() : rows;
+    auto row_at = [&](int64_t i) -> int64_t { return subset ? (int64_t)(*subset)[i] : i; };
     int64_t total = 0;
-    for (int64_t r = 0; r < rows; r++) total += (int64_t)(ptr[r + 1] - ptr[r]) + solve;
+    for (int64_t i = 0; i < nrows; i++) {
+        const int64_t r = row_at(i);
+        total += (int64_t)(ptr[r + 1] - ptr[r]) + solve;
+    }
     const int64_t waves = std::max<int64_t>(1, grid);
     // Parts small enough that the heaviest row spreads over many waves, large
     // enough that the partial-tile traffic stays small next to the gathers.
@@ -22,8 +28,9 @@ Schedule build_schedule(int64_t rows, const int *ptr, int64_t split_len, int64_t
 
     struct Item { Unit u; int64_t cost; };
     std::vector<Item> items;
-    items.reserve(rows + total / s.split_len + 1);
-    for (int64_t r = 0; r < rows; r++) {
+    items.reserve(nrows + total / s.split_len + 1);
+    for (int64_t i = 0; i < nrows; i++) {
+        const int64_t r = row_at(i);
         const int64_t n = ptr[r + 1] - ptr[r];
         if (n <= s.split_len) {
             items.push_back({{(int)r, ptr[r], ptr[r + 1], -1}, n + solve});

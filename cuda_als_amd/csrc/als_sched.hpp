@@ -45,7 +45,9 @@ struct Schedule {
 int64_t row_solve_cost(int prec);
 
 // split_len <= 0 and batch_work <= 0 select sizes from the total work and
-// the number of persistent waves (`grid`).
-Schedule build_schedule(int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work, int grid, int prec);
+// the number of persistent waves (`grid`). subset (nullable): schedule only
+// these rows (ascending), e.g. the rows left after the dual system's.
+Schedule build_schedule(int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work, int grid, int prec,
+                        const std::vector<int> *subset = nullptr);
 
 }  // namespace als

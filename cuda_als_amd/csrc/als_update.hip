@@ -73,6 +73,60 @@ int ALS_CAT(update_lds_bytes_p, ALS_PREC)(int T, int solve, int refine) {
     }
 }
 
+template <int T, int S>
+static const void *dual_one() {
+    if constexpr (Geo<T, ALS_PREC>::DUAL_TD > 0) return (const void *)&dual_kernel<T, ALS_PREC, S>;
+    else return nullptr;
+}
+
+template <int S>
+static const void *dual_for(int T) {
+    switch (T) {
+    case 5: return dual_one<5, S>();
+    case 6: return dual_one<6, S>();
+    case 7: return dual_one<7, S>();
+    case 8: return dual_one<8, S>();
+    case 9: return dual_one<9, S>();
+    default: return nullptr;
+    }
+}
+
+// The dual system exists for T >= 5 (Geo::DUAL_TD > 0), after the plain
+// Cholesky solve or the warm-started one.
+const void *ALS_CAT(dual_kernel_ptr_p, ALS_PREC)(int T, int solve) {
+    return solve == SOLVE_CHOL        ? dual_for<SOLVE_CHOL>(T)
+           : solve == SOLVE_CHOL_WARM ? dual_for<SOLVE_CHOL_WARM>(T)
+                                      : nullptr;
+}
+
+template <int T>
+static int dual_lds_for() {
+    using G = Geo<T, ALS_PREC>;
+    return G::MISC_FLOATS * 4 + G::DUAL_STAGE_BYTES;
+}
+
+int ALS_CAT(dual_lds_bytes_p, ALS_PREC)(int T) {
+    switch (T) {
+    case 5: return dual_lds_for<5>();
+    case 6: return dual_lds_for<6>();
+    case 7: return dual_lds_for<7>();
+    case 8: return dual_lds_for<8>();
+    case 9: return dual_lds_for<9>();
+    default: return 0;
+    }
+}
+
+int ALS_CAT(dual_max_ratings_p, ALS_PREC)(int T) {
+    switch (T) {
+    case 5: return 16 * Geo<5, ALS_PREC>::DUAL_TD;
+    case 6: return 16 * Geo<6, ALS_PREC>::DUAL_TD;
+    case 7: return 16 * Geo<7, ALS_PREC>::DUAL_TD;
+    case 8: return 16 * Geo<8, ALS_PREC>::DUAL_TD;
+    case 9: return 16 * Geo<9, ALS_PREC>::DUAL_TD;
+    default: return 0;
+    }
+}
+
 hipError_t ALS_CAT(launch_update_p, ALS_PREC)(int T, int solve, const UpdateArgs &a, int grid, int lds,
                                              hipStream_t s) {
     const void *k = ALS_CAT(update_kernel_ptr_p, ALS_PREC)(T, solve);
