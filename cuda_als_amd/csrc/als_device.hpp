@@ -92,7 +92,7 @@ template <> __device__ __forceinline__ float from16<PREC_FP16>(uint16_t h) {
 
 // LDS carve-up of one wave (the workgroup is exactly one wave).
 #ifndef ALS_DUAL_TD_MAX
-#define ALS_DUAL_TD_MAX 3
+#define ALS_DUAL_TD_MAX 5
 #endif
 
 template <int T, int E> struct Geo {
@@ -111,10 +111,14 @@ template <int T, int E> struct Geo {
     static constexpr int DUAL_TD = T >= 5 ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
     static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : yt_row(FP) * YT_H * 2;
     static constexpr int DUAL_STAGE_BYTES = DUAL_TD * (DUAL_TD + 1) / 2 * 1024;
+    // LDS of the update kernels (Gram stage) and of the dual kernel (tile
+    // copy); the multi-model kernel, which runs both, takes the larger.
     static constexpr int STAGE_BYTES = GRAM_STAGE_BYTES > DUAL_STAGE_BYTES ? GRAM_STAGE_BYTES : DUAL_STAGE_BYTES;
     // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
     static constexpr int MISC_FLOATS = 32 + 32 + FP + 64;
-    __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + STAGE_BYTES; }
+    __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + GRAM_STAGE_BYTES; }
+    __host__ __device__ static constexpr int dual_lds_bytes() { return MISC_FLOATS * 4 + DUAL_STAGE_BYTES; }
+    __host__ __device__ static constexpr int group_lds_bytes() { return MISC_FLOATS * 4 + STAGE_BYTES; }
 };
 
 // ------------------------------------------------------------------ Gram
@@ -1047,7 +1051,7 @@ template <int TD, int T, int E, bool REFINE>
 __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n, float reg, float (&xc)[T],
                                          float *xs, float *kst) {
     constexpr int NTD = TD * (TD + 1) / 2;
-    static_assert(!REFINE || NTD * 1024 <= Geo<T, E>::STAGE_BYTES, "tile copy must fit the stage");
+    static_assert(!REFINE || NTD * 1024 <= Geo<T, E>::DUAL_STAGE_BYTES, "tile copy must fit the stage");
     int l = lane();
     // Opaque per row: lane masks of this path are formed here, not hoisted
     // out of the row loop where they would stay live across the other path.

@@ -942,7 +942,8 @@ extern "C" int als_group_create(als_engine *const *engines, int count, als_group
     std::stable_sort(g->eng.begin(), g->eng.end(), [](const als_engine *a, const als_engine *b) { return a->T > b->T; });
     g->prec = e0->prec;
     g->solve = SOLVE_CHOL;
-    for (const als_engine *e : g->eng) g->lds = std::max(g->lds, e->lds);
+    // the group kernel runs the dual system inline: room for its tile copy too
+    for (const als_engine *e : g->eng) g->lds = std::max(g->lds, std::max(e->lds, e->dual_lds));
     if (hipSetDevice(e0->cfg.device) != hipSuccess) { group_free(g); return ALS_E_HIP; }
     const void *k = update_group_kernel_ptr(g->prec, g->solve);
     int per_cu = 0;

@@ -102,7 +102,7 @@ const void *ALS_CAT(dual_kernel_ptr_p, ALS_PREC)(int T, int solve) {
 template <int T>
 static int dual_lds_for() {
     using G = Geo<T, ALS_PREC>;
-    return G::MISC_FLOATS * 4 + G::DUAL_STAGE_BYTES;
+    return G::dual_lds_bytes();
 }
 
 int ALS_CAT(dual_lds_bytes_p, ALS_PREC)(int T) {

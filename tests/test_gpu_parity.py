@@ -413,9 +413,10 @@ def test_warm_started_refinement(toy):
 @pytest.mark.parametrize("prec,oprec", [(E.Prec.FP16, O.PREC_FP16), (E.Prec.BF16, O.PREC_BF16)])
 def test_dual_system_for_light_rows(toy, f, prec, oprec):
     """With a 16-bit Gram and one warm-started refinement step, whole rows
-    with fewer than 48 ratings (f >= 64) are solved in the space of their
-    ratings, x = Y^T (Y Y^T + mu I)^-1 r. Every user of the toy workload
-    (at most 40 ratings) and most of its items take that path. One
+    with fewer than 16 min(T - 2, 5) ratings (48 at f = 64, 80 at f = 100)
+    are solved in the space of their ratings, x = Y^T (Y Y^T + mu I)^-1 r.
+    Every user of the toy workload (at most 40 ratings) and most of its
+    items take that path. One
     half-sweep of each side from the oracle's trained state matches the
     FP64 solve of the 16-bit-rounded system within 1e-5 relative to
     max(1, |x|); the rows through the dual system land within ~2e-7."""
@@ -447,12 +448,13 @@ def test_dual_system_for_light_rows(toy, f, prec, oprec):
 def test_dual_system_tile_boundaries(f, prec, refine, warm, oprec, tol):
     """Rows whose rating count puts the augmented row n at every position
     the dual system distinguishes: first and last row of a tile (n = 15, 16,
-    31, 32, 47), the one-rating row, the empty row, and rows just past the
-    dual limit (48, 49, 63) or far past it, which take the primal system.
-    One users half-sweep from random factors against the FP64 oracle
-    (relative to max(1, |x|)); plain solves 1e-4, warm-started 1e-5."""
+    31, 32, 47, 48, 63, 64, 79), the one-rating row, the empty row, and rows
+    just past the dual limit (48 at f = 64, 80 at f = 100) or far past it,
+    which take the primal system. One users half-sweep from random factors
+    against the FP64 oracle (relative to max(1, |x|)); plain solves 1e-4,
+    warm-started 1e-5."""
     rng = np.random.default_rng(17)
-    counts = np.repeat([0, 1, 2, 14, 15, 16, 17, 30, 31, 32, 33, 46, 47, 48, 49, 63, 64, 100], 3)
+    counts = np.repeat([0, 1, 2, 14, 15, 16, 17, 30, 31, 32, 33, 46, 47, 48, 49, 63, 64, 65, 79, 80, 81, 100], 3)
     m, n_items = len(counts), 400
     cols = np.concatenate([np.sort(rng.choice(n_items, c, replace=False)) for c in counts]).astype(np.int32)
     ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
