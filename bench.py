@@ -208,7 +208,10 @@ def main() -> int:
                                                                not a.no_autotune)
         del eng_f
         fp32 = {"sweeps_per_s": a.steps / dt_f, "users_ms": uf, "items_ms": itf, "test_rmse": rmse_f,
-                "failures": int(info_f.failures)}
+                "failures": int(info_f.failures),
+                "solve": ("fp32 MFMA Gram, fp32 Cholesky, no refinement"
+                          + (f"; rows under {info_f.dual_max} ratings through their n x n dual system"
+                             if info_f.dual_max else ""))}
 
     # Dominant kernel: the slower half-sweep of the mixed path (its update kernel plus the dual
     # kernel of its light rows, launched back to back on the same stream and timed together).
@@ -236,7 +239,10 @@ def main() -> int:
         "config": {"workload": f"netflix-shape {d.m}x{d.n}, {d.nnz} train ratings, f={f}, lambda={lam}",
                    "gram": f"{a.prec} MFMA, fp32 accumulate",
                    "solve": (f"fp32 Cholesky + {a.refine} fp64-residual refinement step(s)"
-                             + (", the first from the previous iterate" if a.refine_warm and a.refine else "")),
+                             + (", the first from the previous iterate" if a.refine_warm and a.refine else "")
+                             + (f"; rows under {info_m.dual_max} ratings ({info_m.dual_rows[0]} users, "
+                                f"{info_m.dual_rows[1]} items) through their n x n dual system"
+                                if info_m.dual_max else "")),
                    "users_ms": um, "items_ms": im, "test_rmse_after_steps": rmse_m,
                    "failures": int(info_m.failures), "grid": list(info_m.grid), "lds_bytes": info_m.lds_bytes,
                    "schedule": tune_m, "h2d_upload_s": up_m,

@@ -44,7 +44,7 @@ class AlsInfo(C.Structure):
     _fields_ = [("T", C.c_int), ("ld", C.c_int), ("grid", C.c_int * 2), ("lds_bytes", C.c_int),
                 ("nbatches", C.c_int * 2), ("nunits", C.c_int * 2), ("nsplit_rows", C.c_int * 2),
                 ("split_len", C.c_int64 * 2), ("failures", C.c_int64), ("m", C.c_int64), ("n", C.c_int64),
-                ("nnz", C.c_int64), ("n_test", C.c_int64)]
+                ("nnz", C.c_int64), ("n_test", C.c_int64), ("dual_max", C.c_int), ("dual_rows", C.c_int * 2)]
 
 
 class ScheduleStats(C.Structure):

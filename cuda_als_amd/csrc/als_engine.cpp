@@ -665,6 +665,9 @@ int als_get_info(als_engine *e, als_info *o) {
     o->n = e->n;
     o->nnz = e->nnz;
     o->n_test = e->n_test;
+    o->dual_max = e->dual_max;
+    o->dual_rows[0] = e->side[0].dnunits;
+    o->dual_rows[1] = e->side[1].dnunits;
     return ALS_OK;
 }
 

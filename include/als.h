@@ -96,6 +96,8 @@ typedef struct {
     int64_t split_len[2];/* ratings per part of a split row */
     int64_t failures;    /* rows whose factorisation broke down (x set to 0) */
     int64_t m, n, nnz, n_test;
+    int dual_max;        /* rows with fewer ratings take the dual system (0: none) */
+    int dual_rows[2];    /* rows of each side the dual system takes */
 } als_info;
 
 int als_config_init(als_config *cfg);
