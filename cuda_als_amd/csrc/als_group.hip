@@ -12,8 +12,13 @@
 
 namespace als {
 
+// One function for every T: its register allocation is set by the largest
+// path (T = 9 needs more than half the VGPR file), so the kernel is held to
+// two waves per SIMD; the large-T paths then spill, but every model runs at
+// two waves instead of one (f = 32/64/128 at the netflix shape: 29.8 ms per
+// sweep against 43.8 ms unhinted; three separate engines take 26.4 ms).
 template <int E, int S>
-__global__ __launch_bounds__(WAVE) void update_group_kernel(GroupArgs g) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2, 2))) void update_group_kernel(GroupArgs g) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int total = g.first[g.nmodels];
     for (;;) {

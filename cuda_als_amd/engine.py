@@ -74,9 +74,10 @@ class Engine:
             raise AlsError(rc, lib.als_last_error(self.h).decode())
 
     def close(self) -> None:
-        if getattr(self, "h", None):
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "h", None) and lib is not None:
             lib.als_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()
@@ -216,9 +217,9 @@ class Group:
         self._check(lib.als_group_sweep(self.h))
 
     def close(self) -> None:
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.als_group_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()
