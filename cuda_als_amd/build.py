@@ -57,7 +57,7 @@ def build_lib() -> Path:
     jobs: list[list[str]] = []
     objs: list[Path] = []
     for p in (0, 1, 2):
-        for unit in ("als_update", "als_group"):
+        for unit in ("als_update",):
             o = OBJ / f"{unit}_p{p}.o"
             jobs.append([HIPCC, *HIP_FLAGS, f"-DALS_PREC={p}", "-c", str(CSRC / f"{unit}.hip"), "-o", str(o)])
             objs.append(o)

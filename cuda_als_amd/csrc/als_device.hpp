@@ -111,14 +111,11 @@ template <int T, int E> struct Geo {
     static constexpr int DUAL_TD = T >= 5 ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
     static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : yt_row(FP) * YT_H * 2;
     static constexpr int DUAL_STAGE_BYTES = DUAL_TD * (DUAL_TD + 1) / 2 * 1024;
-    // LDS of the update kernels (Gram stage) and of the dual kernel (tile
-    // copy); the multi-model kernel, which runs both, takes the larger.
-    static constexpr int STAGE_BYTES = GRAM_STAGE_BYTES > DUAL_STAGE_BYTES ? GRAM_STAGE_BYTES : DUAL_STAGE_BYTES;
+    // LDS of the update kernels (Gram stage) and of the dual kernel (tile copy)
     // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
     static constexpr int MISC_FLOATS = 32 + 32 + FP + 64;
     __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + GRAM_STAGE_BYTES; }
     __host__ __device__ static constexpr int dual_lds_bytes() { return MISC_FLOATS * 4 + DUAL_STAGE_BYTES; }
-    __host__ __device__ static constexpr int group_lds_bytes() { return MISC_FLOATS * 4 + STAGE_BYTES; }
 };
 
 // ------------------------------------------------------------------ Gram
@@ -1204,10 +1201,7 @@ __device__ __forceinline__ void dual_pick(const UpdateArgs &a, int64_t rs, int n
 }
 
 // Every work unit of queue batch b of one model.
-// DUAL_INLINE: light rows take the dual system inside this batch loop (the
-// multi-model kernel); the update kernels leave them to dual_kernel, so the
-// dual code does not weigh on their register allocation.
-template <int T, int E, int S, bool DUAL_INLINE = false>
+template <int T, int E, int S>
 __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds) {
     using G = Geo<T, E>;
     const int l = lane();
@@ -1223,27 +1217,7 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
             float x[T];
 #pragma unroll
             for (int J = 0; J < T; ++J) x[J] = 0.f;
-            bool dual = false;
-            if constexpr (DUAL_INLINE && (S == SOLVE_CHOL || S == SOLVE_CHOL_WARM) && G::DUAL_TD > 0) {
-                // Whole rows under 16 * DUAL_TD ratings: the dual system,
-                // refined once where the solve carries a refinement step.
-                if (n > 0 && n < 16 * G::DUAL_TD && w.w < 0 && !a.Lsave) {
-                    const float reg = (a.lambda_rows ? a.lambda_rows[row] : a.lambda) * (float)n;
-                    float *xsd = reinterpret_cast<float *>(lds) + 64;
-                    float *kst = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);
-                    dual_pick<1, G::DUAL_TD, T, E, S == SOLVE_CHOL_WARM>(a, w.y, n, reg, x, xsd, kst);
-                    dual = true;
-                    bool finite = true;
-#pragma unroll
-                    for (int J = 0; J < T; ++J) finite = finite && isfinite(x[J]);
-                    if (__any(!finite)) {
-                        if (l == 0) atomicAdd(a.fail_count, 1);
-#pragma unroll
-                        for (int J = 0; J < T; ++J) x[J] = 0.f;
-                    }
-                }
-            }
-            if (n > 0 && !dual) {
+            if (n > 0) {
                 gram_row<T, E>(a, w.y, w.z - w.y, acc, lds);
                 if (w.w >= 0) {
                     // Split row: publish this part's tiles; the last part to

@@ -86,31 +86,6 @@ hipError_t launch_dual(int T, int prec, int solve, const UpdateArgs &a, int grid
     return hipLaunchKernel(k, dim3(grid), dim3(WAVE), args, (size_t)lds, s);
 }
 
-const void *update_group_kernel_ptr_p0(int solve);
-const void *update_group_kernel_ptr_p1(int solve);
-const void *update_group_kernel_ptr_p2(int solve);
-hipError_t launch_update_group_p0(int solve, const GroupArgs &g, int grid, int lds, hipStream_t s);
-hipError_t launch_update_group_p1(int solve, const GroupArgs &g, int grid, int lds, hipStream_t s);
-hipError_t launch_update_group_p2(int solve, const GroupArgs &g, int grid, int lds, hipStream_t s);
-
-const void *update_group_kernel_ptr(int prec, int solve) {
-    switch (prec) {
-    case PREC_FP32: return update_group_kernel_ptr_p0(solve);
-    case PREC_BF16: return update_group_kernel_ptr_p1(solve);
-    case PREC_FP16: return update_group_kernel_ptr_p2(solve);
-    default: return nullptr;
-    }
-}
-
-hipError_t launch_update_group(int prec, int solve, const GroupArgs &g, int grid, int lds, hipStream_t s) {
-    switch (prec) {
-    case PREC_FP32: return launch_update_group_p0(solve, g, grid, lds, s);
-    case PREC_BF16: return launch_update_group_p1(solve, g, grid, lds, s);
-    case PREC_FP16: return launch_update_group_p2(solve, g, grid, lds, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 // ------------------------------------------------------------------ RMSE
 // Reference: calculate_square_error (als_model.cu:62-89) + cublasSasum +
 // host sqrt (:1744-1764). Sixteen lanes share one rating: each loads float4

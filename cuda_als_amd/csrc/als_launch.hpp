@@ -46,17 +46,7 @@ struct UpdateArgs {
 
 // Several models (different f) on one rating matrix, one launch: batch b of
 // the combined queue is batch b - first[k] of model k (first[k] <= b < first[k+1]).
-constexpr int MAX_MODELS = 4;
-// A group launch runs up to two schedules per model (the rows of the f x f
-// systems, and the light rows the dual system takes): one entry each.
-constexpr int MAX_ENTRIES = 2 * MAX_MODELS;
-struct GroupArgs {
-    UpdateArgs m[MAX_ENTRIES];
-    int T[MAX_ENTRIES];
-    int first[MAX_ENTRIES + 1];
-    int nmodels;                // entries in use
-    int *queue;                 // [0] next batch, [1] waves that left
-};
+constexpr int MAX_MODELS = 4;  // engines per group (als_group_*)
 
 struct AdjointArgs {
     const int *ptr;             // CSR of the side that was updated
@@ -107,7 +97,6 @@ int update_lds_bytes(int T, int prec, int solve, int refine);
 hipError_t launch_update(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds_bytes,
                          hipStream_t s);
 const void *update_kernel_ptr(int T, int prec, int solve);
-hipError_t launch_update_group(int prec, int solve, const GroupArgs &g, int grid, int lds_bytes, hipStream_t s);
 // Light rows through the dual system, one launch per half-sweep after the
 // update kernel (null / 0 where the variant has no dual system: f < 64, CG,
 // classic refinement). dual_max_ratings: rows below it take the dual system.
@@ -115,7 +104,6 @@ const void *dual_kernel_ptr(int T, int prec, int solve);
 int dual_lds_bytes(int T, int prec);
 int dual_max_ratings(int T, int prec);
 hipError_t launch_dual(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds_bytes, hipStream_t s);
-const void *update_group_kernel_ptr(int prec, int solve);
 hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s);
 hipError_t launch_cg_batch(int T, const CgBatchArgs &a, hipStream_t s);
 hipError_t launch_ybar(int T, const YbarArgs &a, hipStream_t s);

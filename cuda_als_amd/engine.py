@@ -193,8 +193,9 @@ def cg_update_x(A, x, b, cg_iters: int = 6, tol: float = 1e-4) -> None:
 
 class Group:
     """Several engines (models of different f) over one rating matrix,
-    updated by one kernel launch per half-sweep (als_group_*). The engines
-    stay usable on their own and must outlive the group."""
+    updated by one graph launch per half-sweep, each model's kernels on a
+    stream of their own (als_group_*). The engines stay usable on their own
+    and must outlive the group."""
 
     def __init__(self, engines: list[Engine]):
         self.engines = list(engines)
@@ -202,8 +203,8 @@ class Group:
         h = C.c_void_p()
         rc = lib.als_group_create(arr, len(self.engines), C.byref(h))
         if rc != 0:
-            raise AlsError(rc, "als_group_create failed (same data, precision and device; Cholesky without "
-                               "refinement; f >= 16)")
+            raise AlsError(rc, "als_group_create failed (1 to 4 distinct engines over the same rating matrix "
+                               "and device)")
         self.h = h
 
     def _check(self, rc: int) -> None:

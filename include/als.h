@@ -190,13 +190,15 @@ int als_autotune(als_engine *e, int sides, als_tune_result *out);
 int als_set_schedule(als_engine *e, int side, int64_t split_len, int64_t batch_work, int restage_max);
 
 /* Model groups (no reference counterpart; the reference trains one model per
- * run): up to 4 engines with different f over the same rating matrix (same
- * m, n, nnz, Gram precision and device; Cholesky solve without refinement;
- * f >= 16) updated by ONE
- * persistent kernel launch per half-sweep on the first engine's stream.
- * Each model's result equals its own als_update exactly (same work units,
- * same per-row arithmetic). The engines stay owned by the caller and must
- * outlive the group. */
+ * run): up to 4 engines over the same rating matrix (same m, n, nnz and
+ * device; any f, precision and solver) updated by ONE graph launch per
+ * half-sweep: each model's own kernels, forked onto a stream per model so
+ * they run concurrently, joined back into the stream of the engine with
+ * the largest f. The launch is ordered after each engine's pending work and
+ * before its later work. The graph is captured on first use and again
+ * whenever a schedule or buffer it holds changes. Each model's result
+ * equals its own als_update exactly (the same kernels on the same work).
+ * The engines stay owned by the caller and must outlive the group. */
 typedef struct als_group als_group;
 int als_group_create(als_engine *const *engines, int count, als_group **out);
 void als_group_destroy(als_group *g);

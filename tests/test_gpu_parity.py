@@ -324,14 +324,16 @@ def test_autotune_keeps_state_and_results():
         assert np.max(np.abs(V0 - V1)) <= 2e-4 * max(1.0, np.abs(V0).max())
 
 
-@pytest.mark.parametrize("prec,refine", [(E.Prec.BF16, 0), (E.Prec.FP16, 0)])
-def test_group_single_launch_equals_separate_runs(prec, refine):
+@pytest.mark.parametrize("prec,refine,warm", [(E.Prec.BF16, 0, False), (E.Prec.FP16, 0, False),
+                                              (E.Prec.FP16, 1, True)])
+def test_group_single_launch_equals_separate_runs(prec, refine, warm):
     """Three models, f = 32 / 64 / 128, on one ml10M-shaped matrix (scaled),
-    updated by ONE launch per half-sweep: every model's factors equal its own
-    separate run bit for bit."""
+    updated by ONE graph launch per half-sweep (each model's kernels on its
+    own stream): every model's factors equal its own separate run bit for
+    bit."""
     d = ca.scaled(ca.ml10m_shape(), 0.1).generate()
     fs = (32, 64, 128)
-    cfgs = [E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=refine) for f in fs]
+    cfgs = [E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=refine, refine_warm=warm) for f in fs]
     sep = [E.Engine(c, d) for c in cfgs]
     grp = [E.Engine(c, d) for c in cfgs]
     for e in sep + grp:
@@ -351,18 +353,32 @@ def test_group_single_launch_equals_separate_runs(prec, refine):
         e.close()
 
 
-def test_group_rejects_mismatched_models(toy):
-    e1 = E.Engine(E.make_config(32, prec=E.Prec.FP16), toy)
-    e2 = E.Engine(E.make_config(64, prec=E.Prec.BF16), toy)
-    e3 = E.Engine(E.make_config(8, prec=E.Prec.FP16), toy)
-    e4 = E.Engine(E.make_config(48, prec=E.Prec.FP16, refine=1), toy)
+def test_group_mixed_models_and_rejections(toy):
+    """Models of any f, precision and solver group together (here FP16 f=32,
+    BF16 f=64, f=8, and CG) and each equals its own run; engines over
+    another rating matrix, a repeated engine or more than four are refused."""
+    cfgs = [E.make_config(32, prec=E.Prec.FP16), E.make_config(64, prec=E.Prec.BF16),
+            E.make_config(8, prec=E.Prec.FP16), E.make_config(48, solve=E.SolveType.CUMF_ALS_CG_FP32)]
+    sep = [E.Engine(c, toy) for c in cfgs]
+    grp = [E.Engine(c, toy) for c in cfgs]
+    for e in sep + grp:
+        e.init_factors()
+    with E.Group(grp) as g:
+        g.sweep()
+    for e in sep:
+        e.sweep()
+    for es, eg in zip(sep, grp):
+        assert np.array_equal(es.get_factors(0), eg.get_factors(0))
+        assert np.array_equal(es.get_factors(1), eg.get_factors(1))
+    other = ca.scaled(ca.ml10m_shape(), 0.02).generate()
+    eo = E.Engine(E.make_config(32, prec=E.Prec.FP16), other)
     with pytest.raises(ca.AlsError):
-        E.Group([e1, e2])              # different Gram precision
+        E.Group([grp[0], eo])          # another rating matrix
     with pytest.raises(ca.AlsError):
-        E.Group([e1, e3])              # f < 16
+        E.Group([grp[0], grp[0]])      # the same engine twice
     with pytest.raises(ca.AlsError):
-        E.Group([e1, e4])              # refinement
-    for e in (e1, e2, e3, e4):
+        E.Group(grp + [eo])            # five engines
+    for e in sep + grp + [eo]:
         e.close()
 
 
