@@ -18,8 +18,9 @@ from oracle import als_oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(d, f, solve, prec, sweeps, refine=0, lam=0.05, graph=False, batch_work=0):
-    cfg = E.make_config(f, lam, sweeps, solve, prec, refine=refine, use_graph=graph, batch_work=batch_work)
+def run_gpu(d, f, solve, prec, sweeps, refine=0, lam=0.05, graph=False, batch_work=0, warm=False):
+    cfg = E.make_config(f, lam, sweeps, solve, prec, refine=refine, use_graph=graph, batch_work=batch_work,
+                        refine_warm=warm)
     with E.Engine(cfg, d) as e:
         e.init_factors()
         r = e.train(sweeps)
@@ -224,10 +225,15 @@ def test_empty_rows_get_zero_factors():
         assert np.max(np.abs(Xg - Xn)) <= 1e-4 * max(1.0, np.abs(Xn).max())
 
 
-def test_graph_replay_matches_direct(toy):
-    _, _, r1, _ = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.FP16, 4, refine=1)
-    _, _, r2, _ = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.FP16, 4, refine=1, graph=True)
+@pytest.mark.parametrize("f,warm", [(16, False), (100, True)])
+def test_graph_replay_matches_direct(toy, f, warm):
+    """Each sweep replayed from the captured graph (with the RMSE and host
+    synchronisation between replays) equals the direct launches; at f = 100
+    the graph holds the dual kernel's launches too."""
+    U1, V1, r1, _ = run_gpu(toy, f, E.SolveType.CHOL, E.Prec.FP16, 4, refine=1, warm=warm)
+    U2, V2, r2, _ = run_gpu(toy, f, E.SolveType.CHOL, E.Prec.FP16, 4, refine=1, graph=True, warm=warm)
     assert np.array_equal(r1, r2)
+    assert np.array_equal(U1, U2) and np.array_equal(V1, V2)
 
 
 def test_schedule_invariance(toy):
