@@ -1026,10 +1026,12 @@ extern "C" int als_group_update(als_group *g, int sd) {
         // launches on its stream, join back into s0
         GCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
         int rc = ALS_OK;
+        const als_engine *failed = nullptr;
         hipError_t st = hipEventRecord(g->fork, s0);
         for (int k = 0; st == hipSuccess && rc == ALS_OK && k < nm; k++) {
             st = hipStreamWaitEvent(g->streams[k], g->fork, 0);
             if (st == hipSuccess) rc = launch_own_side(g->eng[k], sd, g->streams[k]);
+            if (rc != ALS_OK) failed = g->eng[k];
             if (st == hipSuccess && rc == ALS_OK) st = hipEventRecord(g->join[k], g->streams[k]);
             if (st == hipSuccess && rc == ALS_OK) st = hipStreamWaitEvent(s0, g->join[k], 0);
         }
@@ -1037,7 +1039,7 @@ extern "C" int als_group_update(als_group *g, int sd) {
         const hipError_t se = hipStreamEndCapture(s0, &graph);
         if (rc != ALS_OK) {
             if (graph) (void)hipGraphDestroy(graph);
-            return gset_err(g, rc, "group capture: " + g->eng[0]->err);
+            return gset_err(g, rc, "group capture: " + failed->err);
         }
         if (st != hipSuccess || se != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
