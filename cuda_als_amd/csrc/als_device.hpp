@@ -44,6 +44,15 @@ __host__ __device__ constexpr int tile_id(int I, int J) { return I * (I + 1) / 2
 
 __device__ __forceinline__ int lane() { return (int)threadIdx.x; }
 
+// The lane index recomputed where it is used (two VALU instructions): in a
+// loop that holds most registers, the compiler would otherwise keep
+// threadIdx.x in a spill slot and reload it from memory every iteration.
+__device__ __forceinline__ int lane_here() {
+    int v;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(v));
+    return v;
+}
+
 __device__ __forceinline__ float bcast_lane(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
@@ -219,6 +228,107 @@ __device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int
 // the accumulator of the transposed product, acc[tile_id(I,J)][r] =
 // G[16J + 4q + r][16I + c] = G[16I + c][16J + 4q + r] (I > J), the layout
 // the register Cholesky works in.
+// 16-bit Gram of one row when one gather task per lane covers a chunk
+// (4 * nseg <= 64, f <= 127), software-pipelined: the gathers of chunk k+1
+// are issued before the MFMAs of chunk k and land in registers while they
+// run, and the indices and ratings of chunk k+2 are loaded meanwhile. The
+// stage, the task layout and the MFMA order are those of the plain loop,
+// so the sums are the same bit for bit. Stage rows 8s + e (e < 8) sit at
+// 9s + e and rows 16I + c at 18I + yt_row(c), so each lane addresses its
+// writes and operand reads from one base with constant offsets.
+template <int T, int E>
+__device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs, int n,
+                                                 f32x4 (&acc)[Geo<T, E>::NT], char *lds) {
+    using G = Geo<T, E>;
+    const int l = lane();
+    const int f = a.f;
+    int *ibuf = reinterpret_cast<int *>(lds);
+    float *rbuf = reinterpret_cast<float *>(lds) + 32;
+    uint16_t *Yt = reinterpret_cast<uint16_t *>(lds + G::MISC_FLOATS * 4);
+    const int nseg = (f + 7) >> 3;
+    const bool active = l < 4 * nseg;  // task (g, s) of lane l
+    const int g = l / nseg, s = l - g * nseg;
+    uint16_t *wbase = Yt + 9 * s * G::YT_H + 8 * g;
+    const uint16_t *rbase = Yt + G::yt_row(l & 15) * G::YT_H + 8 * (l >> 4);
+    const int64_t soff = 8 * s;
+    if (n <= 0) return;
+    // Raw indices and ratings of the chunk at k0, loaded by every lane (lane
+    // l and l + 32 alike) at positions clamped into the row, so the loads
+    // are unconditional; the positions past the row's end are masked when
+    // the chunk is published.
+    auto load_ids = [&](int k0, int &id_raw, float &v_raw) {
+        const int64_t p = rs + min(k0 + (lane_here() & 31), n - 1);
+        id_raw = a.idx[p];
+        v_raw = a.val[p];
+    };
+    // Publishes the chunk at k0 (ibuf, rbuf: both halves of the wave write
+    // the same values) and gathers its eight rows per active lane into d
+    // (slots past the row's end read the zero row).
+    u32x4 d[8];
+    auto gather = [&](int k0, int id_raw, float v_raw) {
+        const int l31 = lane_here() & 31;
+        const bool in = k0 + l31 < n;
+        ibuf[l31] = in ? id_raw : -1;
+        rbuf[l31] = in ? v_raw : 0.f;
+        wave_sync();
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int id = ibuf[8 * g + e];
+                const uint16_t *src = (id >= 0 ? a.Ys + (int64_t)id * G::FP : a.zero16) + soff;
+                d[e] = *reinterpret_cast<const u32x4 *>(src);
+            }
+        }
+    };
+    int nid;
+    float nval;
+    load_ids(0, nid, nval);
+    gather(0, nid, nval);
+    load_ids(CHUNK, nid, nval);
+    for (int k0 = 0; k0 < n; k0 += CHUNK) {
+        if (active) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                u32x4 lo, hi;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const uint32_t x0 = d[2 * w][v], x1 = d[2 * w + 1][v];
+                    lo[w] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);  // low halves of x0, x1
+                    hi[w] = __builtin_amdgcn_perm(x1, x0, 0x07060302u);  // high halves
+                }
+                *reinterpret_cast<u32x4 *>(wbase + (2 * v) * G::YT_H) = lo;
+                *reinterpret_cast<u32x4 *>(wbase + (2 * v + 1) * G::YT_H) = hi;
+            }
+        }
+        if (const int lh = lane_here(); lh < CHUNK) Yt[G::yt_row(f) * G::YT_H + lh] = to16<E>(rbuf[lh]);
+        wave_sync();
+        u32x4 F[T];
+#pragma unroll
+        for (int I = 0; I < T; ++I) F[I] = *reinterpret_cast<const u32x4 *>(rbase + 18 * I * G::YT_H);
+        if (k0 + CHUNK < n) {
+            // The operand reads above are issued before the next chunk's
+            // transposes write the stage, and LDS keeps a wave's operations
+            // in order; ibuf and rbuf are read before they are rewritten.
+            gather(k0 + CHUNK, nid, nval);
+            load_ids(k0 + 2 * CHUNK, nid, nval);
+        }
+#pragma unroll
+        for (int I = 0; I < T; ++I)
+#pragma unroll
+            for (int J = 0; J <= I; ++J) {
+                if constexpr (E == PREC_BF16)
+                    acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, F[J]), __builtin_bit_cast(bf16x8, F[I]), acc[tile_id(I, J)],
+                        0, 0, 0);
+                else
+                    acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(f16x8, F[J]), __builtin_bit_cast(f16x8, F[I]), acc[tile_id(I, J)],
+                        0, 0, 0);
+            }
+        wave_sync();
+    }
+}
+
 template <int T, int E>
 __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                                          f32x4 (&acc)[Geo<T, E>::NT], char *lds) {
@@ -228,6 +338,11 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
 #pragma unroll
     for (int t = 0; t < G::NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     stage_clear<T, E>(a.f, stage);
+    if constexpr (E != PREC_FP32 && T <= 8) {
+        // f <= 16 T - 1 <= 127: at most 16 segments, one task per lane
+        gram_row16_piped<T, E>(a, rs, n, acc, lds);
+        return;
+    }
     for (int k0 = 0; k0 < n; k0 += CHUNK) {
         stage_chunk<T, E>(a, rs, k0, min(CHUNK, n - k0), lds);
         if constexpr (E == PREC_FP32) {
