@@ -497,3 +497,44 @@ def test_dual_system_tile_boundaries(f, prec, refine, warm, oprec, tol):
     print({int(c): float(e) for c, e in zip(counts[::3], err.reshape(-1, 3).max(axis=1))})
     assert not Ug[counts == 0].any()
     assert err.max() <= tol
+
+
+@pytest.mark.parametrize("f", [16, 63, 100, 127])
+@pytest.mark.parametrize("prec,oprec", [(E.Prec.FP16, O.PREC_FP16), (E.Prec.BF16, O.PREC_BF16)])
+@pytest.mark.parametrize("split_len", [0, 40])
+def test_gram_chunk_boundaries(f, prec, oprec, split_len):
+    """The 16-bit Gram walks a row in 32-rating chunks with the next chunk's
+    gathers in flight (f <= 127). Rows end at every chunk position that
+    loop distinguishes: one rating, a chunk's last and first slot (31, 32,
+    33, ...), one to six chunks, an odd and an even chunk count; f = 63
+    and 127 fill all 64 lanes with gather tasks, f = 16 leaves most idle.
+    split_len 40 cuts the longer rows into parts of 40 ratings (a full and
+    a short chunk per part) summed by the last part. At f = 100 and 127 the
+    rows under 80 ratings take the dual system instead. One users
+    half-sweep from random factors against the FP64 oracle on the same
+    16-bit-rounded inputs, relative to max(1, |x|): 1e-4."""
+    rng = np.random.default_rng(23)
+    counts = np.repeat([1, 31, 32, 33, 63, 64, 65, 95, 96, 97, 127, 128, 129, 160, 191, 192, 193, 250], 2)
+    m, n_items = len(counts), 600
+    cols = np.concatenate([np.sort(rng.choice(n_items, c, replace=False)) for c in counts]).astype(np.int32)
+    ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    vals = rng.integers(1, 6, cols.size).astype(np.float32)
+    rows = np.repeat(np.arange(m), counts)
+    d = ca.from_arrays(m, n_items, ptr, cols, vals, rows[:5], cols[:5], vals[:5])
+    U0 = (0.3 * rng.standard_normal((m, f))).astype(np.float32)
+    V0 = (0.3 * rng.standard_normal((n_items, f))).astype(np.float32)
+    cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, split_len=split_len or (1 << 30))
+    with E.Engine(cfg, d) as e:
+        e.set_factors(0, U0)
+        e.set_factors(1, V0)
+        e.update(ca.ALS_SIDE_USERS)
+        Ug = e.get_factors(0)
+        info = e.info()
+        assert info.failures == 0
+    if split_len:
+        assert info.nsplit_rows[0] >= 20
+    Un, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec), d.csr_ptr, d.csr_idx, d.csr_val, V0,
+                     U0.astype(np.float64))
+    err = np.abs(Ug - Un).max(axis=1) / np.maximum(1.0, np.abs(Un).max(axis=1))
+    print(f, prec, split_len, {int(c): float(x) for c, x in zip(counts[::2], err.reshape(-1, 2).max(axis=1))})
+    assert err.max() <= 1e-4
