@@ -1458,9 +1458,9 @@ __global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
 
 // Light rows through the dual system: whole rows with fewer than
 // 16 * DUAL_TD ratings (and empty rows, x = 0), the units of their own
-// schedule, launched after the update kernel of the same half-sweep. With
-// only 1-3 tiles live it needs far fewer registers than the update kernel
-// and runs at a higher occupancy.
+// schedule, launched beside the update kernel of the same half-sweep. It
+// carries none of the primal Gram and solve code, and the update kernel
+// none of its code.
 template <int T, int E, int S>
 __global__ __launch_bounds__(WAVE) void dual_kernel(UpdateArgs a) {
     using G = Geo<T, E>;

@@ -52,6 +52,11 @@ struct als_engine {
     int dual_max = 0, dual_lds = 0, dual_grid = 0;  // dual kernel: rows below dual_max ratings (0: none)
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
+    // The dual kernel runs beside the update kernel on a stream of its own,
+    // forked from and joined back into `stream` with these events.
+    hipStream_t dual_stream = nullptr;
+    hipEvent_t dual_fork = nullptr, dual_join = nullptr;
+    bool dual_beside = true;  // false: ALS_DUAL_SEQUENTIAL=1 in the environment at als_create
     std::string err;
     int num_cu = 0;
 
@@ -62,7 +67,7 @@ struct als_engine {
     float *t_val = nullptr;
     float *U = nullptr, *V = nullptr;  // FP32 factors, row stride ld
     uint16_t *Us = nullptr, *Vs = nullptr;  // 16-bit shadows (mixed precision)
-    int *queue = nullptr;              // 2 ints
+    int *queue = nullptr;              // 4 ints: update kernel's queue pair, dual kernel's
     int *fail = nullptr;               // 1 int
     double *partials = nullptr, *dsum = nullptr;
     int rmse_blocks = 1024;
@@ -238,15 +243,32 @@ UpdateArgs dual_args(const Side &s, UpdateArgs a) {
     return a;
 }
 
-// Launch one half-sweep: the update kernel over its schedule, then the dual
-// kernel over the light rows. When the factorisation is saved for the
-// adjoint (Lsave), the light rows go through the update kernel too: the
+// Launch one half-sweep: the update kernel over its schedule on st and the
+// dual kernel over the light rows beside it on the engine's dual stream
+// (forked from st before and joined into st after, also under graph
+// capture). The two write disjoint rows of X and pop their own work queues;
+// running them together lets the dual kernel's waves fill the SIMDs the
+// update kernel's last waves leave idle. With beside = false (the model
+// group's capture, which forks its own streams) the dual kernel follows
+// the update kernel on st. When the factorisation is saved for the adjoint
+// (Lsave), the light rows go through the update kernel too, after it: the
 // adjoint needs every row's f x f factor.
 int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
-                const float *lambda_rows, float *Lsave, hipStream_t st) {
+                const float *lambda_rows, float *Lsave, hipStream_t st, bool beside = true) {
     const Side &s = e->side[sd];
     if (s.rows == 0) return ALS_OK;
     const UpdateArgs a = side_args(e, sd, Y, Ys, X, Xs, lambda_rows, Lsave);
+    if (beside && e->dual_beside && !Lsave && s.nbatches > 0 && s.dnbatches > 0) {
+        UpdateArgs d = dual_args(s, a);
+        d.queue = e->queue + 2;
+        HIPCHK(e, hipEventRecord(e->dual_fork, st));
+        HIPCHK(e, hipStreamWaitEvent(e->dual_stream, e->dual_fork, 0));
+        HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, st));
+        HIPCHK(e, launch_dual(e->T, e->prec, e->solve, d, e->dual_grid, e->dual_lds, e->dual_stream));
+        HIPCHK(e, hipEventRecord(e->dual_join, e->dual_stream));
+        HIPCHK(e, hipStreamWaitEvent(st, e->dual_join, 0));
+        return ALS_OK;
+    }
     if (s.nbatches > 0) HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, st));
     if (s.dnbatches > 0) {
         const UpdateArgs d = dual_args(s, a);
@@ -257,10 +279,10 @@ int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float
 }
 
 // One half-sweep of the engine's own factors on stream st.
-int launch_own_side(als_engine *e, int sd, hipStream_t st) {
+int launch_own_side(als_engine *e, int sd, hipStream_t st, bool beside = true) {
     if (sd == ALS_SIDE_USERS)
-        return launch_side(e, 0, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr, st);
-    return launch_side(e, 1, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr, st);
+        return launch_side(e, 0, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr, st, beside);
+    return launch_side(e, 1, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr, st, beside);
 }
 
 int update_side(als_engine *e, int sd) {
@@ -415,14 +437,21 @@ int als_create(const als_config *cfg, als_engine **out) {
     }
     if (!rc && e->solve == SOLVE_CHOL_REFINE)
         rc = dev_alloc(e, &e->scratch, (size_t)e->side[0].grid * (e->T * (e->T + 1) / 2) * 256);
-    if (!rc) rc = dev_alloc(e, &e->queue, 2);
+    // Per-dispatch hardware counters (rocprofv3 --pmc) need the two kernels
+    // of a half-sweep one after the other: ALS_DUAL_SEQUENTIAL=1 keeps them so.
+    if (const char *v = std::getenv("ALS_DUAL_SEQUENTIAL")) e->dual_beside = std::atoi(v) == 0;
+    if (!rc && (hipStreamCreateWithFlags(&e->dual_stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&e->dual_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&e->dual_join, hipEventDisableTiming) != hipSuccess))
+        rc = set_err(e, ALS_E_HIP, "cannot create the dual kernel's stream");
+    if (!rc) rc = dev_alloc(e, &e->queue, 4);
     if (!rc) rc = dev_alloc(e, &e->fail, 1);
     if (!rc) rc = dev_alloc(e, &e->partials, e->rmse_blocks);
     if (!rc) rc = dev_alloc(e, &e->dsum, 1);
     if (!rc) rc = dev_alloc(e, &e->zero32, e->ld);
     if (!rc) rc = dev_alloc(e, &e->zero16, e->ld);
     if (!rc) {
-        if (hipMemsetAsync(e->queue, 0, 2 * sizeof(int), e->stream) != hipSuccess ||
+        if (hipMemsetAsync(e->queue, 0, 4 * sizeof(int), e->stream) != hipSuccess ||
             hipMemsetAsync(e->fail, 0, sizeof(int), e->stream) != hipSuccess ||
             hipMemsetAsync(e->zero32, 0, e->ld * sizeof(float), e->stream) != hipSuccess ||
             hipMemsetAsync(e->zero16, 0, e->ld * sizeof(uint16_t), e->stream) != hipSuccess ||
@@ -442,6 +471,10 @@ void als_destroy(als_engine *e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
     for (void *p : e->allocs) (void)hipFree(p);
+    if (e->dual_stream) (void)hipStreamSynchronize(e->dual_stream);
+    if (e->dual_fork) (void)hipEventDestroy(e->dual_fork);
+    if (e->dual_join) (void)hipEventDestroy(e->dual_join);
+    if (e->dual_stream) (void)hipStreamDestroy(e->dual_stream);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
     delete e;
 }
@@ -1030,7 +1063,7 @@ extern "C" int als_group_update(als_group *g, int sd) {
         hipError_t st = hipEventRecord(g->fork, s0);
         for (int k = 0; st == hipSuccess && rc == ALS_OK && k < nm; k++) {
             st = hipStreamWaitEvent(g->streams[k], g->fork, 0);
-            if (st == hipSuccess) rc = launch_own_side(g->eng[k], sd, g->streams[k]);
+            if (st == hipSuccess) rc = launch_own_side(g->eng[k], sd, g->streams[k], false);
             if (rc != ALS_OK) failed = g->eng[k];
             if (st == hipSuccess && rc == ALS_OK) st = hipEventRecord(g->join[k], g->streams[k]);
             if (st == hipSuccess && rc == ALS_OK) st = hipStreamWaitEvent(s0, g->join[k], 0);

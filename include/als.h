@@ -112,7 +112,10 @@ int als_config_from_reference(int f, float lambda, int iters, int calculate_vtvs
 int als_create(const als_config *cfg, als_engine **out);
 void als_destroy(als_engine *e);
 const char *als_last_error(const als_engine *e);
-/* hip_stream: a hipStream_t (NULL selects the engine's own stream). */
+/* hip_stream: a hipStream_t (NULL selects the engine's own stream). A
+ * half-sweep may also run a kernel on a second stream of the engine, forked
+ * from this stream and joined back into it, so work enqueued on this stream
+ * is ordered before and after it as usual. */
 int als_set_stream(als_engine *e, void *hip_stream);
 
 /* Host arrays in the reference layout (cuda_sparse_matrix.h:29-37); copied. */
