@@ -236,6 +236,23 @@ def test_graph_replay_matches_direct(toy, f, warm):
     assert np.array_equal(U1, U2) and np.array_equal(V1, V2)
 
 
+def test_dual_kernel_beside_equals_after(monkeypatch):
+    """The dual kernel runs beside the update kernel on a second stream; with
+    ALS_DUAL_SEQUENTIAL=1 at engine creation it runs after it. The two write
+    disjoint rows, so the factors and the RMSE trajectory are identical, in
+    direct launches and in a captured graph. netflix-shaped matrix (scaled),
+    f = 100, FP16 Gram, one warm-started step: both kernels have work."""
+    d = ca.scaled(ca.netflix_shape(), 0.02).generate()
+    out = []
+    for seq, graph in (("1", False), ("0", False), ("0", True)):
+        monkeypatch.setenv("ALS_DUAL_SEQUENTIAL", seq)
+        U, V, r, info = run_gpu(d, 100, E.SolveType.CHOL, E.Prec.FP16, 3, refine=1, warm=True, graph=graph)
+        assert info.failures == 0 and info.dual_rows[0] > 0 and info.nunits[0] > info.dual_rows[0]
+        out.append((U, V, r))
+    for U, V, r in out[1:]:
+        assert np.array_equal(U, out[0][0]) and np.array_equal(V, out[0][1]) and np.array_equal(r, out[0][2])
+
+
 def test_schedule_invariance(toy):
     """Batch size changes which wave takes which row, never the result."""
     _, _, r1, i1 = run_gpu(toy, 32, E.SolveType.CHOL, E.Prec.FP32, 2, batch_work=1)
