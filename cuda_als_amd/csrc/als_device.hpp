@@ -272,10 +272,15 @@ __device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs
         rbuf[l31] = in ? v_raw : 0.f;
         wave_sync();
         if (active) {
+            // the task's eight indices in two 16-byte reads, so the loads
+            // wait for LDS once rather than once each
+            const int gi = lane_here() / nseg;  // = g, recomputed rather than kept
+            const int4 i0 = *reinterpret_cast<const int4 *>(ibuf + 8 * gi);
+            const int4 i1 = *reinterpret_cast<const int4 *>(ibuf + 8 * gi + 4);
+            const int ids[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int id = ibuf[8 * g + e];
-                const uint16_t *src = (id >= 0 ? a.Ys + (int64_t)id * G::FP : a.zero16) + soff;
+                const uint16_t *src = (ids[e] >= 0 ? a.Ys + (int64_t)ids[e] * G::FP : a.zero16) + soff;
                 d[e] = *reinterpret_cast<const u32x4 *>(src);
             }
         }
