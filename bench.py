@@ -3,10 +3,13 @@
 Workload: 480,189 users x 17,770 items, 99,072,112 train ratings (1,408,395
 held out), f = 100, lambda = 0.05, seeded synthetic data (seed 42). A step is
 one full sweep (all users from V, then all items from U) with the data
-already resident in HBM. The mixed-precision path (FP16 Gram on MFMA with
-FP32 accumulation, FP32 Cholesky, FP64-residual refinement) is the measured
-line; the plain FP32 path (FP32 MFMA Gram, FP32 Cholesky) is timed beside it
-for the speed ratio and the test-RMSE comparison.
+already resident in HBM, timed as replays of the captured sweep graph. The
+mixed-precision path (FP16 Gram on MFMA with FP32 accumulation, FP32
+Cholesky, FP64-residual refinement) is the measured line; the plain FP32 path
+(FP32 MFMA Gram, FP32 Cholesky) is timed beside it for the speed ratio, and
+so is the reference's production solver, FP32 CG with 6 iterations. Each
+path's test RMSE over 10 sweeps from the reference init is compared with the
+FP64 oracle's trajectory of the same workload (tests/golden fixture).
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 
@@ -48,8 +51,13 @@ def parse_traffic(kernel_tag: str):
     return None
 
 
-def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True):
-    """Runs warmup + timed sweeps; returns (seconds, per-half kernel ms, rmse, info)."""
+def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True, oracle=None):
+    """Warmup, then `steps` sweeps timed as whole-sweep graph replays (the
+    headline figure), then `steps` sweeps launched half by half with HIP
+    events between the halves on the engine's stream (per-half times). With
+    `oracle` (the FP64 trajectory fixture and the name of its solver), a
+    last pass from the reference init compares the test RMSE of every sweep
+    with it. Returns a dict."""
     stream = torch.cuda.Stream()          # default priority; a real (non-null) stream handle
     torch.cuda.set_stream(stream)
     t = time.perf_counter()
@@ -63,11 +71,14 @@ def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True):
         tune = eng.autotune()
         log(f"[{tag}] autotune: {tune}")
     for _ in range(warmup):
-        eng.sweep()
-    torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        eng.sweep()                       # the first one captures the sweep graph
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.sweep()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
     for s in range(steps):
         ev[s][0].record(stream)
         eng.update(ca.ALS_SIDE_USERS)
@@ -75,14 +86,55 @@ def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True):
         eng.update(ca.ALS_SIDE_ITEMS)
         ev[s][2].record(stream)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
     users_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
     items_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
     rmse = eng.rmse(ca.ALS_RMSE_TEST)
     info = eng.info()
-    log(f"[{tag}] {steps} sweeps in {dt:.3f}s -> {steps / dt:.2f} sweeps/s; users {users_ms:.3f} ms, "
-        f"items {items_ms:.3f} ms per half; test RMSE {rmse:.6f}; failures {info.failures}")
-    return eng, dt, users_ms, items_ms, rmse, info, upload_s, tune
+    log(f"[{tag}] {steps} sweeps (graph) in {dt:.3f}s -> {steps / dt:.2f} sweeps/s; half by half: users "
+        f"{users_ms:.3f} ms, items {items_ms:.3f} ms; test RMSE {rmse:.6f}; failures {info.failures}")
+    out = {"eng": eng, "dt": dt, "users_ms": users_ms, "items_ms": items_ms, "rmse": rmse, "info": info,
+           "upload_s": upload_s, "tune": tune, "vs_oracle": None}
+    if oracle is not None:
+        fix, name = oracle
+        eng.init_factors()
+        got = []
+        for _ in range(fix["sweeps"]):
+            eng.sweep()
+            got.append(eng.rmse(ca.ALS_RMSE_TEST))
+        want = fix[f"rmse_{name}"]
+        out["vs_oracle"] = {"oracle": f"FP64 {name}", "sweeps": fix["sweeps"],
+                            "max_abs_delta": max(abs(a - b) for a, b in zip(got, want)),
+                            "final_delta": got[-1] - want[-1]}
+        log(f"[{tag}] test RMSE vs the FP64 oracle ({name}), per sweep: "
+            + " ".join(f"{a - b:+.1e}" for a, b in zip(got, want)))
+    return out
+
+
+def oracle_fixture(d, f):
+    """The FP64 oracle's 10-sweep trajectory of this workload
+    (tests/golden/netflix_trajectory.json), if it was made from the same
+    generated arrays and f."""
+    import hashlib
+    p = ROOT / "tests" / "golden" / "netflix_trajectory.json"
+    if not p.exists():
+        return None
+    fix = json.loads(p.read_text())
+    if fix.get("f") != f:
+        return None
+    h = hashlib.sha256()
+    for a in (d.csr_ptr, d.csr_idx, d.csr_val, d.csc_ptr, d.csc_idx, d.csc_val, d.t_row, d.t_col, d.t_val):
+        h.update(a.tobytes())
+    return fix if h.hexdigest() == fix["data_sha256"] else None
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(d, f, lam, budget_s=20.0):
@@ -121,7 +173,7 @@ def cpu_baseline(d, f, lam, budget_s=20.0):
     s_one, pr1 = sweep_seconds(budget_s / 2)
     O.lib.orc_set_num_threads(cores)
     return {"value": 1.0 / s_all, "unit": "sweeps/s", "cores": cores, "kind": "port",
-            "value_1core": 1.0 / s_one,
+            "value_1core": 1.0 / s_one, "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": (f"FP64 oracle Cholesky update of {pr['users'][1]} user rows and {pr['items'][1]} item "
                        f"rows drawn uniformly with replacement ({cores} threads; {pr1['users'][1]} + "
                        f"{pr1['items'][1]} at 1 thread), per-row time scaled to all {d.m} users + {d.n} items")}
@@ -186,36 +238,47 @@ def main() -> int:
     log(f"generated {d.m} x {d.n}, {d.nnz} train / {d.n_test} test ratings in {time.perf_counter() - t:.1f}s")
     f, lam = a.f, 0.05
     prec = E.Prec.FP16 if a.prec == "fp16" else E.Prec.BF16
+    fix = oracle_fixture(d, f) if a.scale == 1.0 else None
+    log("FP64 oracle trajectory fixture:", "found" if fix else "not applicable")
+
+    def run(cfg, tag, oname):
+        r = time_path(torch, E, ca, d, cfg, a.steps, a.warmup, tag, not a.no_autotune,
+                      (fix, oname) if fix else None)
+        del r["eng"]
+        return r
 
     mixed_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine,
-                              refine_warm=bool(a.refine_warm))
-    eng_m, dt_m, um, im, rmse_m, info_m, up_m, tune_m = time_path(torch, E, ca, d, mixed_cfg, a.steps, a.warmup,
-                                                                  "mixed", not a.no_autotune)
-    del eng_m
+                              refine_warm=bool(a.refine_warm), use_graph=True)
+    M = run(mixed_cfg, "mixed", "chol")
+    info_m = M["info"]
     classic = None
     if a.refine_warm and a.refine > 0 and not a.no_classic:
-        c_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine)
-        eng_c, dt_c, uc, ic, rmse_c, info_c, _, _ = time_path(torch, E, ca, d, c_cfg, a.steps, a.warmup,
-                                                              "mixed-classic", not a.no_autotune)
-        del eng_c
-        classic = {"sweeps_per_s": a.steps / dt_c, "users_ms": uc, "items_ms": ic, "test_rmse": rmse_c,
-                   "failures": int(info_c.failures),
+        c_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, prec, refine=a.refine, use_graph=True)
+        C = run(c_cfg, "mixed-classic", "chol")
+        classic = {"sweeps_per_s": a.steps / C["dt"], "users_ms": C["users_ms"], "items_ms": C["items_ms"],
+                   "test_rmse": C["rmse"], "failures": int(C["info"].failures), "vs_oracle": C["vs_oracle"],
                    "solve": f"fp32 Cholesky + {a.refine} fp64-residual refinement step(s) of the fresh solve"}
-    fp32 = None
+    fp32 = cg6 = None
     if not a.no_fp32:
-        fp32_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, E.Prec.FP32)
-        eng_f, dt_f, uf, itf, rmse_f, info_f, _, _ = time_path(torch, E, ca, d, fp32_cfg, a.steps, a.warmup, "fp32",
-                                                               not a.no_autotune)
-        del eng_f
-        fp32 = {"sweeps_per_s": a.steps / dt_f, "users_ms": uf, "items_ms": itf, "test_rmse": rmse_f,
-                "failures": int(info_f.failures),
+        fp32_cfg = E.make_config(f, lam, a.steps, E.SolveType.CHOL, E.Prec.FP32, use_graph=True)
+        F = run(fp32_cfg, "fp32", "chol")
+        fp32 = {"sweeps_per_s": a.steps / F["dt"], "users_ms": F["users_ms"], "items_ms": F["items_ms"],
+                "test_rmse": F["rmse"], "failures": int(F["info"].failures), "vs_oracle": F["vs_oracle"],
                 "solve": ("fp32 MFMA Gram, fp32 Cholesky, no refinement"
-                          + (f"; rows under {info_f.dual_max} ratings through their n x n dual system"
-                             if info_f.dual_max else ""))}
+                          + (f"; rows under {F['info'].dual_max} ratings through their n x n dual system"
+                             if F["info"].dual_max else ""))}
+        cg_cfg = E.make_config(f, lam, a.steps, E.SolveType.CUMF_ALS_CG_FP32, E.Prec.FP32, cg_iters=6, cg_tol=1e-4,
+                               use_graph=True)
+        G = run(cg_cfg, "fp32-cg6", "cg6")
+        cg6 = {"sweeps_per_s": a.steps / G["dt"], "users_ms": G["users_ms"], "items_ms": G["items_ms"],
+               "test_rmse": G["rmse"], "failures": int(G["info"].failures), "vs_oracle": G["vs_oracle"],
+               "solve": "fp32 MFMA Gram, cuMF CG recurrence: 6 iterations, warm start, exit on r.r < 1e-4 "
+                        "(the reference's production solver)"}
 
     # Dominant kernel: the slower half-sweep of the mixed path (its update kernel plus the dual
-    # kernel of its light rows, launched back to back on the same stream and timed together).
-    half, ms, rows_nnz = ("items", im, d.nnz) if im >= um else ("users", um, d.nnz)
+    # kernel of its light rows, launched side by side from the engine's stream and timed together).
+    um, im = M["users_ms"], M["items_ms"]
+    half, ms = ("items", im) if im >= um else ("users", um)
     gram_flops = float(d.nnz) * f * (f + 1)           # lower triangle incl. diagonal, 2 flop per MAC
     achieved = gram_flops / (ms * 1e-3) / 1e12
     # The launch also factors one (f+1)x(f+1) system per row in FP32: (f+1)^3/3 MACs. Its time
@@ -226,15 +289,18 @@ def main() -> int:
     roof_ms = (gram_flops / (PEAK_FP16_TFLOPS * 1e12) + solve_flops / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP16_TFLOPS, "traffic": parse_traffic(f"update_{half}"),
+                "traffic_source": ("not measured in this run: HBM bytes per launch of the same kernels from "
+                                   "profiles/traffic_update_netflix_shape.md (rocprofv3 --pmc FETCH_SIZE and "
+                                   "WRITE_SIZE, scaled by a plain copy's counters)"),
                 "kernel": (f"{half} half-sweep: update_kernel (16-bit Gram + Cholesky) and dual_kernel "
-                           "(light rows), timed together"),
+                           "(light rows), timed together with HIP events on the engine's stream"),
                 "kernel_ms": ms, "flops_per_launch": gram_flops,
                 "solve_fp32_flops_per_launch": solve_flops, "roofline_ms_gram_fp16_plus_solve_fp32": roof_ms,
                 "frac_time": roof_ms / ms}
 
     line = {
-        "metric": METRIC, "value": a.steps / dt_m, "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": dt_m / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": a.steps / M["dt"], "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": M["dt"] / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": a.prec, "data": "synthetic (seeded, seed 42)",
         "config": {"workload": f"netflix-shape {d.m}x{d.n}, {d.nnz} train ratings, f={f}, lambda={lam}",
                    "gram": f"{a.prec} MFMA, fp32 accumulate",
@@ -243,14 +309,19 @@ def main() -> int:
                              + (f"; rows under {info_m.dual_max} ratings ({info_m.dual_rows[0]} users, "
                                 f"{info_m.dual_rows[1]} items) through their n x n dual system"
                                 if info_m.dual_max else "")),
-                   "users_ms": um, "items_ms": im, "test_rmse_after_steps": rmse_m,
+                   "timing": "whole sweeps replayed from the captured sweep graph; per-half times from a "
+                             "second pass launched half by half",
+                   "users_ms": um, "items_ms": im, "test_rmse_after_steps": M["rmse"],
                    "failures": int(info_m.failures), "grid": list(info_m.grid), "lds_bytes": info_m.lds_bytes,
-                   "schedule": tune_m, "h2d_upload_s": up_m,
-                   "sweeps_per_s_incl_h2d": a.steps / (dt_m + up_m)},
+                   "schedule": M["tune"], "h2d_upload_s": M["upload_s"],
+                   "sweeps_per_s_incl_h2d": a.steps / (M["dt"] + M["upload_s"])},
         "fp32_path": fp32,
+        "fp32_cg6_path": cg6,
         "mixed_classic_refine": classic,
-        "speedup_vs_fp32": (a.steps / dt_m) / fp32["sweeps_per_s"] if fp32 else None,
-        "test_rmse_delta_vs_fp32": abs(rmse_m - fp32["test_rmse"]) if fp32 else None,
+        "speedup_vs_fp32": (a.steps / M["dt"]) / fp32["sweeps_per_s"] if fp32 else None,
+        "test_rmse_delta_vs_fp32": abs(M["rmse"] - fp32["test_rmse"]) if fp32 else None,
+        "test_rmse_delta_vs_oracle": ({"mixed": M["vs_oracle"], "fp32": fp32 and fp32["vs_oracle"],
+                                       "fp32_cg6": cg6 and cg6["vs_oracle"]} if fix else None),
         "roofline": roofline,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(d, f, lam),
     }

@@ -86,6 +86,7 @@ SIGNATURES = {
     "als_set_factors": (C.c_int, [P, C.c_int, P, C.c_int64, C.c_int]),
     "als_get_factors": (C.c_int, [P, C.c_int, P, C.c_int64, C.c_int]),
     "als_update": (C.c_int, [P, C.c_int]),
+    "als_update_rows": (C.c_int, [P, C.c_int, C.c_int64, C.c_int64, C.c_int]),
     "als_sweep": (C.c_int, [P]),
     "als_train": (C.c_int, [P, C.c_int, P]),
     "als_rmse": (C.c_int, [P, C.c_int, F64P]),
@@ -95,6 +96,9 @@ SIGNATURES = {
     "als_device_factors": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "als_forward_dev": (C.c_int, [P, C.c_int, P, P, P, C.c_int]),
     "als_adjoint_dev": (C.c_int, [P, C.c_int, P, P, P, P, P, P]),
+    "als_factor_save_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),
+    "als_forward_dev_ws": (C.c_int, [P, C.c_int, P, P, P, P, C.c_int64]),
+    "als_adjoint_dev_ws": (C.c_int, [P, C.c_int, P, P, P, C.c_int64, P, P, P, P]),
     "als_schedule_preview": (C.c_int, [C.c_int64, P, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(ScheduleStats), P, C.c_int64]),
     "als_autotune": (C.c_int, [P, C.c_int, C.POINTER(TuneResult)]),
@@ -106,6 +110,7 @@ SIGNATURES = {
     "als_group_sweep": (C.c_int, [P]),
     "als_cg_update_x": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
     "als_cg_update_x_fp16": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
+    "als_cg_update_x_tt_fp16": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
     # als_data.h
     "als_synth_generate": (C.c_int, [C.POINTER(SynthSpec), C.POINTER(P)]),
     "als_dataset_from_csr": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_int64, P, P, P, C.POINTER(P)]),

@@ -4,9 +4,12 @@
 
 solves, for every row u of `side`, x_u = A_u^-1 b_u with
 A_u = sum y_i y_i^T + lam_u n_u I and b_u = sum r_ui y_i, on the GPU engine
-(als_forward_dev). Its backward is the engine's hand-written adjoint
-(als_adjoint_dev): xbar = A^-1 g on the saved Cholesky factor,
-dL/dlam_u = -n_u xbar_u . x_u, dL/dY through A and b.
+(als_forward_dev_ws). Its backward is the engine's hand-written adjoint
+(als_adjoint_dev_ws): xbar = A^-1 g on the saved Cholesky factor,
+dL/dlam_u = -n_u xbar_u . x_u, dL/dY through A and b. Every forward call
+keeps its own factorisation in a tensor of the autograd context, so
+half-sweeps chained in one graph (users then items, unrolled sweeps) each
+differentiate against their own factor.
 
 Tensors: lam (rows,), Y (cols, f) float32 on the engine's device; X is
 (rows, f). The engine works on a padded row stride (als_info.ld); padding is
@@ -34,14 +37,16 @@ class HalfSweep(torch.autograd.Function):
         Yp = F.pad(Y.detach().float(), (0, ld - f)).contiguous()
         lamc = lam.detach().float().contiguous()
         Xp = torch.zeros(rows, ld, device=Y.device, dtype=torch.float32)
-        engine.forward_dev(side, Yp.data_ptr(), Xp.data_ptr(), lamc.data_ptr(), save=True)
+        nbytes = engine.factor_save_bytes(side)
+        Ls = torch.empty(nbytes // 4, device=Y.device, dtype=torch.float32)
+        engine.forward_dev_ws(side, Yp.data_ptr(), Xp.data_ptr(), lamc.data_ptr(), Ls.data_ptr(), nbytes)
         ctx.engine, ctx.side, ctx.f, ctx.ld = engine, side, f, ld
-        ctx.save_for_backward(lamc, Yp, Xp)
+        ctx.save_for_backward(lamc, Yp, Xp, Ls)
         return Xp[:, :f].clone()
 
     @staticmethod
     def backward(ctx, gX):
-        lamc, Yp, Xp = ctx.saved_tensors
+        lamc, Yp, Xp, Ls = ctx.saved_tensors
         engine, side, f, ld = ctx.engine, ctx.side, ctx.f, ctx.ld
         stream = torch.cuda.current_stream()
         engine.set_stream(stream.cuda_stream)
@@ -49,8 +54,8 @@ class HalfSweep(torch.autograd.Function):
         xbar = torch.zeros_like(Xp)
         lbar = torch.zeros_like(lamc)
         ybar = torch.zeros_like(Yp)
-        engine.adjoint_dev(side, Yp.data_ptr(), Xp.data_ptr(), g.data_ptr(), xbar.data_ptr(), lbar.data_ptr(),
-                           ybar.data_ptr())
+        engine.adjoint_dev_ws(side, Yp.data_ptr(), Xp.data_ptr(), Ls.data_ptr(), Ls.numel() * 4, g.data_ptr(),
+                              xbar.data_ptr(), lbar.data_ptr(), ybar.data_ptr())
         return lbar, ybar[:, :f], None, None
 
 

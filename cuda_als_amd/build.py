@@ -61,7 +61,9 @@ def build_lib() -> Path:
             o = OBJ / f"{unit}_p{p}.o"
             jobs.append([HIPCC, *HIP_FLAGS, f"-DALS_PREC={p}", "-c", str(CSRC / f"{unit}.hip"), "-o", str(o)])
             objs.append(o)
-    for src in ("als_kernels.hip", "als_engine.cpp", "als_sched.cpp"):
+    # every other HIP unit of csrc/, plus the host units that call the HIP runtime
+    hip_units = sorted(p.name for p in CSRC.glob("*.hip") if p.name != "als_update.hip")
+    for src in (*hip_units, "als_engine.cpp", "als_sched.cpp"):
         o = OBJ / (Path(src).stem + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         jobs.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(o)])

@@ -124,6 +124,11 @@ template <int T, int E> struct Geo {
     // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
     static constexpr int MISC_FLOATS = 32 + 32 + FP + 64;
     __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + GRAM_STAGE_BYTES; }
+    // the classic-refinement variant restages FP32 rows into the same stage
+    static constexpr int FP32_STAGE_BYTES = CHUNK * YR * 4;
+    __host__ __device__ static constexpr int refine_lds_bytes() {
+        return MISC_FLOATS * 4 + (GRAM_STAGE_BYTES > FP32_STAGE_BYTES ? GRAM_STAGE_BYTES : FP32_STAGE_BYTES);
+    }
     __host__ __device__ static constexpr int dual_lds_bytes() { return MISC_FLOATS * 4 + DUAL_STAGE_BYTES; }
 };
 
@@ -903,10 +908,10 @@ __device__ __forceinline__ void lanes_to_colform(const double (&acc)[NJ], int f,
 }
 
 // Residual of one whole row without A: r = sum_i y_i (r_i - y_i . x) - reg x,
-// the row's ratings restaged 32 at a time, y_i and r_i exactly as the Gram
-// saw them (16-bit rounded on the mixed path), reg the FP32 lambda * n the
-// diagonal got. Products and sums in FP64. x column form; r column form
-// (zero past f).
+// the row's ratings restaged 32 at a time in precision E (the refinement
+// restages the FP32 master rows, E = PREC_FP32, whatever the Gram's
+// precision), reg the FP32 lambda * n the diagonal got. Products and sums in
+// FP64. x column form; r column form (zero past f).
 template <int T, int E>
 __device__ __forceinline__ void residual_mf(const UpdateArgs &a, int64_t rs, int n, float reg, const float (&xc)[T],
                                             float (&rc)[T], char *lds) {
@@ -1405,7 +1410,10 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     if constexpr (S == SOLVE_CHOL_REFINE) {
                         for (int s = warm ? 1 : 0; s < a.refine; ++s) {
                             float rc[T];
-                            if (restage) residual_mf<T, E>(a, w.y, n, reg, x, rc, lds);
+                            // restaged from the FP32 master rows: the residual of the
+                            // unrounded system, which the correction solves then
+                            // converge to (the 16-bit Gram's factor preconditions them)
+                            if (restage) residual_mf<T, PREC_FP32>(a, w.y, n, reg, x, rc, lds);
                             else residual_copy<T>(Ascr, f, x, rc);
                             correct<T>(acc, f, rc, x);
                         }

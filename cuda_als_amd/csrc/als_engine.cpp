@@ -70,9 +70,10 @@ struct als_engine {
     int *queue = nullptr;              // 4 ints: update kernel's queue pair, dual kernel's
     int *fail = nullptr;               // 1 int
     double *partials = nullptr, *dsum = nullptr;
-    int rmse_blocks = 1024;
-    float *Lsave = nullptr;
+    int rmse_blocks = RMSE_BLOCKS;
+    float *Lsave = nullptr;            // saved factorisation of als_forward_dev (adjoint)
     size_t Lsave_bytes = 0;
+    int Lsave_side = -1;               // side it belongs to (-1: none)
     float *scratch = nullptr;          // per-wave A copy for refinement
     float *zero32 = nullptr;           // one zero row (16T floats) for unused chunk slots
     uint16_t *zero16 = nullptr;
@@ -129,19 +130,8 @@ int check_csr(als_engine *e, int64_t rows, int64_t cols, int64_t nnz, const int 
     return ALS_OK;
 }
 
-// (Re)build one side's work schedule and upload it; replaces any previous one.
-int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work) {
-    // With a dual kernel, the light rows get a schedule of their own.
-    std::vector<int> heavy, light;
-    if (e->dual_max > 0) {
-        for (int64_t r = 0; r < rows; r++) (ptr[r + 1] - ptr[r] < e->dual_max ? light : heavy).push_back((int)r);
-    }
-    const bool dual = e->dual_max > 0;
-    Schedule sc = build_schedule(rows, ptr, (int)split_len, (int)batch_work, s.grid, e->prec, dual ? &heavy : nullptr);
-    Schedule sd;
-    if (dual && !light.empty()) sd = build_schedule(rows, ptr, (int64_t)1 << 30, 0, e->dual_grid, e->prec, &light);
-    static_assert(sizeof(Unit) == sizeof(int4) && sizeof(Split) == sizeof(int4), "unit layout");
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+// Frees one side's schedule buffers.
+void free_schedule(als_engine *e, Side &s) {
     dev_free(e, s.units);
     dev_free(e, s.batch);
     dev_free(e, s.splits);
@@ -154,7 +144,30 @@ int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr, int64_t spl
     s.partial = nullptr;
     s.dunits = nullptr;
     s.dbatch = nullptr;
+    s.nbatches = s.nunits = s.nsplits = 0;
     s.dnbatches = s.dnunits = 0;
+}
+
+// (Re)build one side's work schedule over rows [rb, re) and upload it;
+// replaces any previous one.
+int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr, int64_t split_len, int64_t batch_work,
+               int64_t rb = 0, int64_t re = -1) {
+    if (re < 0) re = rows;
+    // With a dual kernel, the light rows get a schedule of their own.
+    std::vector<int> heavy, light;
+    const bool ranged = rb > 0 || re < rows;
+    if (e->dual_max > 0 || ranged) {
+        for (int64_t r = rb; r < re; r++)
+            (e->dual_max > 0 && ptr[r + 1] - ptr[r] < e->dual_max ? light : heavy).push_back((int)r);
+    }
+    const bool dual = e->dual_max > 0;
+    Schedule sc = build_schedule(rows, ptr, (int)split_len, (int)batch_work, s.grid, e->prec,
+                                 dual || ranged ? &heavy : nullptr);
+    Schedule sd;
+    if (dual && !light.empty()) sd = build_schedule(rows, ptr, (int64_t)1 << 30, 0, e->dual_grid, e->prec, &light);
+    static_assert(sizeof(Unit) == sizeof(int4) && sizeof(Split) == sizeof(int4), "unit layout");
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    free_schedule(e, s);
     int rc;
     if (!sd.units.empty()) {
         if ((rc = dev_alloc(e, &s.dunits, sd.units.size())) || (rc = dev_alloc(e, &s.dbatch, sd.batch_begin.size())))
@@ -197,9 +210,8 @@ int drop_graph(als_engine *e) {
 
 // Argument block of one half-sweep. X/Y/shadow pointers and lambda are
 // explicit so the same path serves the engine's own factors and caller buffers.
-UpdateArgs side_args(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
+UpdateArgs side_args(als_engine *e, const Side &s, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
                      const float *lambda_rows, float *Lsave) {
-    Side &s = e->side[sd];
     UpdateArgs a{};
     a.ptr = s.ptr;
     a.idx = s.idx;
@@ -253,29 +265,34 @@ UpdateArgs dual_args(const Side &s, UpdateArgs a) {
 // the update kernel on st. When the factorisation is saved for the adjoint
 // (Lsave), the light rows go through the update kernel too, after it: the
 // adjoint needs every row's f x f factor.
-int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
-                const float *lambda_rows, float *Lsave, hipStream_t st, bool beside = true) {
-    const Side &s = e->side[sd];
+int launch_schedule(als_engine *e, const Side &s, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
+                    const float *lambda_rows, float *Lsave, hipStream_t st, bool beside, int solve) {
     if (s.rows == 0) return ALS_OK;
-    const UpdateArgs a = side_args(e, sd, Y, Ys, X, Xs, lambda_rows, Lsave);
+    UpdateArgs a = side_args(e, s, Y, Ys, X, Xs, lambda_rows, Lsave);
+    if (solve == SOLVE_CHOL) a.refine = 0;
     if (beside && e->dual_beside && !Lsave && s.nbatches > 0 && s.dnbatches > 0) {
         UpdateArgs d = dual_args(s, a);
         d.queue = e->queue + 2;
         HIPCHK(e, hipEventRecord(e->dual_fork, st));
         HIPCHK(e, hipStreamWaitEvent(e->dual_stream, e->dual_fork, 0));
-        HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, st));
-        HIPCHK(e, launch_dual(e->T, e->prec, e->solve, d, e->dual_grid, e->dual_lds, e->dual_stream));
+        HIPCHK(e, launch_update(e->T, e->prec, solve, a, s.grid, e->lds, st));
+        HIPCHK(e, launch_dual(e->T, e->prec, solve, d, e->dual_grid, e->dual_lds, e->dual_stream));
         HIPCHK(e, hipEventRecord(e->dual_join, e->dual_stream));
         HIPCHK(e, hipStreamWaitEvent(st, e->dual_join, 0));
         return ALS_OK;
     }
-    if (s.nbatches > 0) HIPCHK(e, launch_update(e->T, e->prec, e->solve, a, s.grid, e->lds, st));
+    if (s.nbatches > 0) HIPCHK(e, launch_update(e->T, e->prec, solve, a, s.grid, e->lds, st));
     if (s.dnbatches > 0) {
         const UpdateArgs d = dual_args(s, a);
-        if (Lsave) HIPCHK(e, launch_update(e->T, e->prec, e->solve, d, s.grid, e->lds, st));
-        else HIPCHK(e, launch_dual(e->T, e->prec, e->solve, d, e->dual_grid, e->dual_lds, st));
+        if (Lsave) HIPCHK(e, launch_update(e->T, e->prec, solve, d, s.grid, e->lds, st));
+        else HIPCHK(e, launch_dual(e->T, e->prec, solve, d, e->dual_grid, e->dual_lds, st));
     }
     return ALS_OK;
+}
+
+int launch_side(als_engine *e, int sd, const float *Y, const uint16_t *Ys, float *X, uint16_t *Xs,
+                const float *lambda_rows, float *Lsave, hipStream_t st, bool beside = true) {
+    return launch_schedule(e, e->side[sd], Y, Ys, X, Xs, lambda_rows, Lsave, st, beside, e->solve);
 }
 
 // One half-sweep of the engine's own factors on stream st.
@@ -320,7 +337,7 @@ int rmse_launch(als_engine *e, int which, double *host_out) {
     a.f = e->cfg.f;
     a.ld = e->ld;
     a.partials = e->partials;
-    HIPCHK(e, launch_rmse(a, e->rmse_blocks, e->dsum, e->stream));
+    HIPCHK(e, launch_rmse(a, e->rmse_blocks, e->dsum, false, e->stream));
     double s = 0.0;
     HIPCHK(e, hipMemcpyAsync(&s, e->dsum, sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
@@ -635,6 +652,40 @@ int als_update(als_engine *e, int sd) {
     return update_side(e, sd);
 }
 
+// Rows [row_begin, row_begin + row_count) of one side from the current
+// factors of the other: a schedule of its own for the range (built, run and
+// freed here), the engine's kernels. exact_solve: the plain Cholesky solve
+// whatever the engine's solver (the reference's LU_solve_U/V always
+// factorises).
+int als_update_rows(als_engine *e, int sd, int64_t row_begin, int64_t row_count, int exact_solve) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "als_set_train has not been called");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    const Side &s0 = e->side[sd];
+    if (row_begin < 0 || row_count < 0 || row_begin + row_count > s0.rows)
+        return set_err(e, ALS_E_INVALID, "row range outside the side's rows");
+    if (row_count == 0) return ALS_OK;
+    drop_graph(e);
+    Side t = s0;  // same arrays, grid and parameters; a schedule of its own
+    t.units = t.splits = nullptr;
+    t.batch = t.counters = nullptr;
+    t.partial = nullptr;
+    t.dunits = nullptr;
+    t.dbatch = nullptr;
+    int rc = build_side(e, t, s0.rows, e->hptr[sd].data(), s0.split_len, s0.batch_work, row_begin,
+                        row_begin + row_count);
+    const int solve = exact_solve ? SOLVE_CHOL : e->solve;
+    if (!rc) {
+        rc = sd == ALS_SIDE_USERS
+                 ? launch_schedule(e, t, e->V, e->Vs, e->U, e->Us, t.lambda_rows, nullptr, e->stream, true, solve)
+                 : launch_schedule(e, t, e->U, e->Us, e->V, e->Vs, t.lambda_rows, nullptr, e->stream, true, solve);
+    }
+    const hipError_t st = hipStreamSynchronize(e->stream);
+    free_schedule(e, t);
+    if (!rc && st != hipSuccess) rc = set_err(e, ALS_E_HIP, std::string("update_rows: ") + hipGetErrorString(st));
+    return rc;
+}
+
 int als_sweep(als_engine *e) {
     if (!e) return ALS_E_INVALID;
     if (!e->have_train) return set_err(e, ALS_E_STATE, "als_set_train has not been called");
@@ -739,12 +790,21 @@ int als_save_libmf(als_engine *e, const char *path) {
     return ALS_OK;
 }
 
-int als_forward_dev(als_engine *e, int sd, const float *d_Y, float *d_X, const float *d_lambda_rows, int save) {
-    if (!e || !d_Y || !d_X) return ALS_E_INVALID;
+}  // extern "C"
+
+namespace {
+
+// Bytes of the saved factorisation of one side: NT tiles of 256 floats per row.
+int64_t factor_save_bytes(const als_engine *e, int sd) {
+    const int64_t rows = sd == 0 ? e->m : e->n;
+    return std::max<int64_t>(1, rows) * (int64_t)(e->T * (e->T + 1) / 2) * 256 * (int64_t)sizeof(float);
+}
+
+int forward_impl(als_engine *e, int sd, const float *d_Y, float *d_X, const float *d_lambda_rows, float *Ls) {
     if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
     if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
     if (e->solve == SOLVE_CG) return set_err(e, ALS_E_UNSUPPORTED, "the adjoint needs the Cholesky solver");
-    const int64_t rows = sd == 0 ? e->m : e->n, cols = sd == 0 ? e->n : e->m;
+    const int64_t cols = sd == 0 ? e->n : e->m;
     int rc = ALS_OK;
     const uint16_t *Ys = nullptr;
     if (e->prec != PREC_FP32) {
@@ -757,32 +817,16 @@ int als_forward_dev(als_engine *e, int sd, const float *d_Y, float *d_X, const f
         HIPCHK(e, launch_to16(e->prec, d_Y, e->tmp16, cols * e->ld, e->stream));
         Ys = e->tmp16;
     }
-    float *Ls = nullptr;
-    if (save) {
-        const size_t np = (size_t)(e->T * (e->T + 1) / 2) * 256;  // factor tiles per row
-        size_t need = (size_t)std::max<int64_t>(1, rows) * np * sizeof(float);
-        if (e->Lsave_bytes < need) {
-            dev_free(e, e->Lsave);
-            if ((rc = dev_alloc(e, &e->Lsave, need / sizeof(float)))) return rc;
-            e->Lsave_bytes = need;
-        }
-        Ls = e->Lsave;
-    }
     return launch_side(e, sd, d_Y, Ys, d_X, nullptr, d_lambda_rows, Ls, e->stream);
 }
 
-int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, const float *d_gbar, float *d_xbar,
-                    float *d_lambda_bar, float *d_ybar) {
-    if (!e || !d_Y || !d_X || !d_gbar || !d_xbar || !d_lambda_bar) return ALS_E_INVALID;
-    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+int adjoint_impl(als_engine *e, int sd, const float *d_Y, const float *d_X, const float *Ls, const float *d_gbar,
+                 float *d_xbar, float *d_lambda_bar, float *d_ybar) {
     const int64_t rows = sd == 0 ? e->m : e->n, cols = sd == 0 ? e->n : e->m;
-    const size_t np = (size_t)(e->T * (e->T + 1) / 2) * 256;
-    if (!e->Lsave || e->Lsave_bytes < (size_t)rows * np * sizeof(float))
-        return set_err(e, ALS_E_STATE, "run als_forward_dev with save_factor first");
     AdjointArgs a{};
     a.ptr = e->side[sd].ptr;
     a.X = d_X;
-    a.Lsave = e->Lsave;
+    a.Lsave = Ls;
     a.Gbar = d_gbar;
     a.Xbar = d_xbar;
     a.lambda_bar = d_lambda_bar;
@@ -804,6 +848,72 @@ int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, c
         HIPCHK(e, launch_ybar(e->T, y, e->stream));
     }
     return ALS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int als_forward_dev(als_engine *e, int sd, const float *d_Y, float *d_X, const float *d_lambda_rows, int save) {
+    if (!e || !d_Y || !d_X) return ALS_E_INVALID;
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    float *Ls = nullptr;
+    if (save) {
+        if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+        const size_t need = (size_t)factor_save_bytes(e, sd);
+        if (e->Lsave_bytes < need) {
+            dev_free(e, e->Lsave);
+            e->Lsave = nullptr;
+            e->Lsave_bytes = 0;
+            int rc = dev_alloc(e, &e->Lsave, need / sizeof(float));
+            if (rc) return rc;
+            e->Lsave_bytes = need;
+        }
+        Ls = e->Lsave;
+    }
+    const int rc = forward_impl(e, sd, d_Y, d_X, d_lambda_rows, Ls);
+    // the engine keeps one saved factorisation: the last saving call's side
+    if (save) e->Lsave_side = rc == ALS_OK ? sd : -1;
+    return rc;
+}
+
+int als_adjoint_dev(als_engine *e, int sd, const float *d_Y, const float *d_X, const float *d_gbar, float *d_xbar,
+                    float *d_lambda_bar, float *d_ybar) {
+    if (!e || !d_Y || !d_X || !d_gbar || !d_xbar || !d_lambda_bar) return ALS_E_INVALID;
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (!e->Lsave || e->Lsave_side != sd)
+        return set_err(e, ALS_E_STATE, "the engine's saved factorisation is not of this side: run als_forward_dev "
+                                       "with save_factor on this side first (or keep one per call with "
+                                       "als_forward_dev_ws / als_adjoint_dev_ws)");
+    return adjoint_impl(e, sd, d_Y, d_X, e->Lsave, d_gbar, d_xbar, d_lambda_bar, d_ybar);
+}
+
+int als_factor_save_bytes(als_engine *e, int sd, int64_t *bytes) {
+    if (!e || !bytes) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    *bytes = factor_save_bytes(e, sd);
+    return ALS_OK;
+}
+
+int als_forward_dev_ws(als_engine *e, int sd, const float *d_Y, float *d_X, const float *d_lambda_rows,
+                       float *d_save, int64_t save_bytes) {
+    if (!e || !d_Y || !d_X || !d_save) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (save_bytes < factor_save_bytes(e, sd))
+        return set_err(e, ALS_E_INVALID, "save buffer smaller than als_factor_save_bytes()");
+    return forward_impl(e, sd, d_Y, d_X, d_lambda_rows, d_save);
+}
+
+int als_adjoint_dev_ws(als_engine *e, int sd, const float *d_Y, const float *d_X, const float *d_save,
+                       int64_t save_bytes, const float *d_gbar, float *d_xbar, float *d_lambda_bar, float *d_ybar) {
+    if (!e || !d_Y || !d_X || !d_save || !d_gbar || !d_xbar || !d_lambda_bar) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (save_bytes < factor_save_bytes(e, sd))
+        return set_err(e, ALS_E_INVALID, "save buffer smaller than als_factor_save_bytes()");
+    return adjoint_impl(e, sd, d_Y, d_X, d_save, d_gbar, d_xbar, d_lambda_bar, d_ybar);
 }
 
 }  // extern "C"
@@ -986,8 +1096,8 @@ std::vector<char> group_signature(als_group *g, int sd) {
     for (als_engine *e : g->eng) {
         const Side &s = e->side[sd];
         const UpdateArgs a = sd == ALS_SIDE_USERS
-                                 ? side_args(e, 0, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr)
-                                 : side_args(e, 1, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr);
+                                 ? side_args(e, s, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr)
+                                 : side_args(e, s, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr);
         put(&a, sizeof(a));
         const int v[] = {e->T, e->prec, e->solve, s.grid, e->lds, s.nbatches, s.dnbatches, e->dual_grid,
                          e->dual_lds};
@@ -1111,13 +1221,14 @@ extern "C" int als_group_sweep(als_group *g) {
 // enqueued on hip_stream (null: the default stream) of the current device,
 // and the call returns once that stream has finished it.
 static int cg_update_x(const float *A, const uint16_t *A16, float *x, const float *b, int batch, int f, int cg_iters,
-                       float tol, void *hip_stream) {
+                       float tol, void *hip_stream, bool round_a16 = false) {
     if (batch < 0 || f < 1 || f > 143 || cg_iters < 0 || !(tol >= 0.f)) return ALS_E_INVALID;
     if (batch == 0) return ALS_OK;
     if ((!A && !A16) || !x || !b) return ALS_E_INVALID;
     CgBatchArgs a{};
     a.A = A;
     a.A16 = A16;
+    a.round_a16 = round_a16 ? 1 : 0;
     a.x = x;
     a.b = b;
     a.batch = batch;
@@ -1138,6 +1249,11 @@ extern "C" int als_cg_update_x(const float *A, float *x, const float *b, int bat
 extern "C" int als_cg_update_x_fp16(const uint16_t *A, float *x, const float *b, int batch, int f, int cg_iters,
                                     float tol, void *hip_stream) {
     return cg_update_x(nullptr, A, x, b, batch, f, cg_iters, tol, hip_stream);
+}
+
+extern "C" int als_cg_update_x_tt_fp16(const float *A, float *x, const float *b, int batch, int f, int cg_iters,
+                                       float tol, void *hip_stream) {
+    return cg_update_x(A, nullptr, x, b, batch, f, cg_iters, tol, hip_stream, true);
 }
 
 extern "C" int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_work, int grid,

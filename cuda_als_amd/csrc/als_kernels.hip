@@ -108,7 +108,10 @@ __global__ __launch_bounds__(RMSE_THREADS) void rmse_kernel(RmseArgs a) {
         for (int j = 4 * sub; j < a.f; j += 64) {
             const f32x4 x = *reinterpret_cast<const f32x4 *>(u + j);
             const f32x4 y = *reinterpret_cast<const f32x4 *>(v + j);
-            d += x[0] * y[0] + x[1] * y[1] + x[2] * y[2] + x[3] * y[3];
+            // columns f.. of the last slice are masked (padding of a caller's rows need not be zero)
+            const float p3 = j + 3 < a.f ? x[3] * y[3] : 0.f, p2 = j + 2 < a.f ? x[2] * y[2] : 0.f;
+            const float p1 = j + 1 < a.f ? x[1] * y[1] : 0.f;
+            d += x[0] * y[0] + p1 + p2 + p3;
         }
 #pragma unroll
         for (int m = 8; m >= 1; m >>= 1) d += __shfl_xor(d, m, 16);
@@ -127,18 +130,23 @@ __global__ __launch_bounds__(RMSE_THREADS) void rmse_kernel(RmseArgs a) {
     }
 }
 
-__global__ __launch_bounds__(WAVE) void sum_partials_kernel(const double *p, int nb, double *out) {
+// out[0] = the sum of the partials; with with_rmse, out[1] = sqrt(sum / n).
+__global__ __launch_bounds__(WAVE) void sum_partials_kernel(const double *p, int nb, double *out, int64_t n,
+                                                            int with_rmse) {
     double s = 0.0;
     for (int i = threadIdx.x; i < nb; i += WAVE) s += p[i];
     s = wave_sum_d(s);
-    if (threadIdx.x == 0) *out = s;
+    if (threadIdx.x == 0) {
+        out[0] = s;
+        if (with_rmse) out[1] = sqrt(s / (double)n);
+    }
 }
 
-hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, hipStream_t s) {
+hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, bool with_rmse, hipStream_t s) {
     rmse_kernel<<<blocks, RMSE_THREADS, 0, s>>>(a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    sum_partials_kernel<<<1, WAVE, 0, s>>>(a.partials, blocks, out);
+    sum_partials_kernel<<<1, WAVE, 0, s>>>(a.partials, blocks, out, a.n, with_rmse ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -198,28 +206,43 @@ __global__ __launch_bounds__(WAVE) void adjoint_solve_kernel(AdjointArgs a) {
 
 // dL/dy_i = sum_{u in Omega(i)} r_ui xbar_u - xbar_u (x_u . y_i) - x_u (xbar_u . y_i)
 // One wave per column i of the rating matrix, walking its CSC list, so every
-// output row has exactly one writer (no atomics).
+// output row has exactly one writer (no atomics). Lane l owns factors
+// l, l + 64, l + 128 (NJ of them: the padded row is 16 T <= 144 wide).
 template <int T>
 __global__ __launch_bounds__(WAVE) void ybar_kernel(YbarArgs a) {
-    constexpr int FP = 16 * T;
+    constexpr int FP = 16 * T, NJ = (FP + WAVE - 1) / WAVE;
     const int l = lane(), f = a.f;
     for (int col = blockIdx.x; col < a.ncols; col += gridDim.x) {
         const int64_t oy = (int64_t)col * FP;
-        const float ylo = l < f ? a.Y[oy + l] : 0.f;
-        const float yhi = WAVE + l < f ? a.Y[oy + WAVE + l] : 0.f;
-        float acl = 0.f, ach = 0.f;
+        float y[NJ], ac[NJ];
+#pragma unroll
+        for (int m = 0; m < NJ; ++m) {
+            const int j = WAVE * m + l;
+            y[m] = j < f ? a.Y[oy + j] : 0.f;
+            ac[m] = 0.f;
+        }
         for (int k = a.cptr[col]; k < a.cptr[col + 1]; ++k) {
             const int64_t ox = (int64_t)a.cidx[k] * FP;
             const float r = a.cval[k];
-            const float xl = l < f ? a.X[ox + l] : 0.f, xh = WAVE + l < f ? a.X[ox + WAVE + l] : 0.f;
-            const float bl = l < f ? a.Xbar[ox + l] : 0.f, bh = WAVE + l < f ? a.Xbar[ox + WAVE + l] : 0.f;
-            const float xy = wave_sum(xl * ylo + xh * yhi);
-            const float by = wave_sum(bl * ylo + bh * yhi);
-            acl += r * bl - bl * xy - xl * by;
-            ach += r * bh - bh * xy - xh * by;
+            float x[NJ], b[NJ], pxy = 0.f, pby = 0.f;
+#pragma unroll
+            for (int m = 0; m < NJ; ++m) {
+                const int j = WAVE * m + l;
+                x[m] = j < f ? a.X[ox + j] : 0.f;
+                b[m] = j < f ? a.Xbar[ox + j] : 0.f;
+                pxy += x[m] * y[m];
+                pby += b[m] * y[m];
+            }
+            const float xy = wave_sum(pxy);
+            const float by = wave_sum(pby);
+#pragma unroll
+            for (int m = 0; m < NJ; ++m) ac[m] += r * b[m] - b[m] * xy - x[m] * by;
         }
-        if (l < FP) a.Ybar[oy + l] = l < f ? acl : 0.f;
-        if (WAVE + l < FP) a.Ybar[oy + WAVE + l] = WAVE + l < f ? ach : 0.f;
+#pragma unroll
+        for (int m = 0; m < NJ; ++m) {
+            const int j = WAVE * m + l;
+            if (j < FP) a.Ybar[oy + j] = j < f ? ac[m] : 0.f;
+        }
     }
 }
 
@@ -239,6 +262,7 @@ __global__ __launch_bounds__(WAVE) void cg_batch_kernel(CgBatchArgs a) {
             if (i >= f || j >= f) return 0.f;
             const int64_t k = base + (int64_t)i * f + j;
             if (a.A16) return (float)__builtin_bit_cast(_Float16, a.A16[k]);
+            if (a.round_a16) return (float)(_Float16)a.A[k];
             return a.A[k];
         };
         f32x4 acc[NT];

@@ -65,6 +65,7 @@ struct AdjointArgs {
 struct CgBatchArgs {
     const float *A;             // FP32 systems, or nullptr when A16 is set
     const uint16_t *A16;        // FP16 systems
+    int round_a16;              // FP32 systems whose entries are rounded to FP16 on load
     float *x;                   // warm start in, solution out
     const float *b;
     int batch;
@@ -107,7 +108,10 @@ hipError_t launch_dual(int T, int prec, int solve, const UpdateArgs &a, int grid
 hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s);
 hipError_t launch_cg_batch(int T, const CgBatchArgs &a, hipStream_t s);
 hipError_t launch_ybar(int T, const YbarArgs &a, hipStream_t s);
-hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, hipStream_t s);
+// RMSE partial sums of a.n ratings over `blocks` blocks into a.partials
+// (blocks doubles), then out[0] = their sum (with_rmse: out[1] = sqrt(sum / n)).
+constexpr int RMSE_BLOCKS = 1024;
+hipError_t launch_rmse(const RmseArgs &a, int blocks, double *out, bool with_rmse, hipStream_t s);
 hipError_t launch_to16(int prec, const float *src, uint16_t *dst, int64_t count, hipStream_t s);
 
 }  // namespace als

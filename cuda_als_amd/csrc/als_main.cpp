@@ -18,8 +18,6 @@
 
 #include "als_model.hpp"
 
-using namespace als_mi355x;
-
 namespace {
 
 std::map<std::string, std::string> parse_ini(const std::string &path, bool &ok) {
@@ -139,10 +137,12 @@ int main(int argc, char **argv) {
                 als_synchronize(model.engine);
                 lg.log("event ALS_UPDATE_V finished", true, now_s() - t0);
                 lg.log("event ALS_ITER finished", true, now_s() - t_it);
+                // the reference's CALC_ITER_RMSE report (als_model.cu:2269-2274)
+                if (it < iters - 1) lg.log("RMSE test: " + std::to_string(model.rsme_test()), true);
             }
             lg.als_iter = 0;
-            if (als_get_factors(model.engine, ALS_SIDE_USERS, model.h_UT.data(), m, f) != ALS_OK ||
-                als_get_factors(model.engine, ALS_SIDE_ITEMS, model.h_VT.data(), n, f) != ALS_OK)
+            if (als_get_factors(model.engine, ALS_SIDE_USERS, model.h_UT, m, f) != ALS_OK ||
+                als_get_factors(model.engine, ALS_SIDE_ITEMS, model.h_VT, n, f) != ALS_OK)
                 throw std::runtime_error(als_last_error(model.engine));
             lg.log("event MF_MODEL_TRAINING finished", true, now_s() - t_train);
             lg.log("final RMSE test: " + std::to_string(model.rsme_test()), true);

@@ -1,60 +1,76 @@
-// C++ mirror of the reference's als_model / host_sparse_matrix on top of the
-// engine's C ABI.
+// C++ mirror of the reference's als_model / host_sparse_matrix and the cuMF
+// CG host entry points, on top of the engine's C ABI.
 #include "als_model.hpp"
 
+#include <algorithm>
 #include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <stdexcept>
 
+#include "als_cg.h"
 #include "als_data.h"
 
-namespace als_mi355x {
-
-namespace {
-
+// ------------------------------------------------------- host_sparse_matrix
 // Reference: host_sparse_matrix::malloc_load_file, cuda_sparse_matrix.cpp:6-27
-// (missing file, empty file and size mismatch are errors).
-template <class T>
-void load_file(const std::string &path, std::vector<T> &dst, size_t count) {
+// (a missing, empty or wrongly sized file is an error; the buffer is owned
+// by the matrix). Plain host memory: the engine copies it to the device once.
+void host_sparse_matrix::malloc_load_file(std::string path, void **host_ptr, size_t buf_size) {
     std::ifstream ifs(path, std::ios::binary | std::ios::ate);
     if (!ifs) throw std::runtime_error(path + ": " + std::strerror(errno));
-    auto size = (size_t)ifs.tellg();
+    const auto size = (size_t)ifs.tellg();
     if (size == 0) throw std::runtime_error(path + " is empty");
-    if (size != count * sizeof(T)) throw std::runtime_error("file size does not match buffer size");
+    if (size != buf_size) throw std::runtime_error("file size does not match buffer size");
     ifs.seekg(0, std::ios::beg);
-    dst.resize(count);
-    if (!ifs.read(reinterpret_cast<char *>(dst.data()), size)) throw std::runtime_error(path + ": read failed");
+    void *buf = std::malloc(buf_size);
+    if (!buf) throw std::runtime_error(path + ": out of host memory");
+    if (!ifs.read(static_cast<char *>(buf), (std::streamsize)buf_size)) {
+        std::free(buf);
+        throw std::runtime_error(path + ": read failed");
+    }
+    std::free(*host_ptr);  // a reload replaces the previous array
+    *host_ptr = buf;
 }
-
-}  // namespace
 
 host_sparse_matrix::host_sparse_matrix(size_t r, size_t c, size_t v) : row_cnt(r), col_cnt(c), val_cnt(v) {}
 
-void host_sparse_matrix::load_csr(const std::string &vals, const std::string &row_ptrs, const std::string &col_idxs) {
+host_sparse_matrix::~host_sparse_matrix() {
+    for (void *p : {(void *)h_csr_coo_vals, (void *)h_csr_row_ptrs, (void *)h_csr_coo_col_idxs,
+                    (void *)h_coo_row_idxs, (void *)h_csc_vals, (void *)h_csc_row_idxs, (void *)h_csc_col_ptrs})
+        std::free(p);
+}
+
+// CSR and COO share the value and column-index arrays: whichever is loaded
+// first reads them (cuda_sparse_matrix.cpp:43-69).
+void host_sparse_matrix::load_csr(std::string vals, std::string row_ptrs, std::string col_idxs) {
     if (!has_coo) {
-        load_file(vals, csr_coo_vals, val_cnt);
-        load_file(col_idxs, csr_coo_col_idxs, val_cnt);
+        malloc_load_file(vals, (void **)&h_csr_coo_vals, val_cnt * sizeof(float));
+        malloc_load_file(col_idxs, (void **)&h_csr_coo_col_idxs, val_cnt * sizeof(int));
     }
-    load_file(row_ptrs, csr_row_ptrs, row_cnt + 1);
+    malloc_load_file(row_ptrs, (void **)&h_csr_row_ptrs, (row_cnt + 1) * sizeof(int));
     has_csr = true;
 }
 
-void host_sparse_matrix::load_csc(const std::string &vals, const std::string &row_idxs, const std::string &col_ptrs) {
-    load_file(vals, csc_vals, val_cnt);
-    load_file(row_idxs, csc_row_idxs, val_cnt);
-    load_file(col_ptrs, csc_col_ptrs, col_cnt + 1);
+void host_sparse_matrix::load_csc(std::string vals, std::string row_idxs, std::string col_ptrs) {
+    malloc_load_file(vals, (void **)&h_csc_vals, val_cnt * sizeof(float));
+    malloc_load_file(row_idxs, (void **)&h_csc_row_idxs, val_cnt * sizeof(int));
+    malloc_load_file(col_ptrs, (void **)&h_csc_col_ptrs, (col_cnt + 1) * sizeof(int));
     has_csc = true;
 }
 
-void host_sparse_matrix::load_coo(const std::string &vals, const std::string &row_idxs, const std::string &col_idxs) {
+void host_sparse_matrix::load_coo(std::string vals, std::string row_idxs, std::string col_idxs) {
     if (!has_csr) {
-        load_file(vals, csr_coo_vals, val_cnt);
-        load_file(col_idxs, csr_coo_col_idxs, val_cnt);
+        malloc_load_file(vals, (void **)&h_csr_coo_vals, val_cnt * sizeof(float));
+        malloc_load_file(col_idxs, (void **)&h_csr_coo_col_idxs, val_cnt * sizeof(int));
     }
-    load_file(row_idxs, coo_row_idxs, val_cnt);
+    malloc_load_file(row_idxs, (void **)&h_coo_row_idxs, val_cnt * sizeof(int));
     has_coo = true;
 }
 
+// ---------------------------------------------------------------- als_model
 void als_model::check(int rc, const char *what) {
     if (rc != ALS_OK)
         throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " +
@@ -87,30 +103,45 @@ void als_model::setup(host_sparse_matrix &tr, host_sparse_matrix &te, const als_
     int rc = als_create(&cfg, &engine);
     if (rc != ALS_OK) throw std::runtime_error("als_create failed (" + std::to_string(rc) + ")");
     try {
-        check(als_set_train(engine, m, n, (int64_t)tr.val_cnt, tr.csr_row_ptrs.data(), tr.csr_coo_col_idxs.data(),
-                            tr.csr_coo_vals.data(), tr.csc_col_ptrs.data(), tr.csc_row_idxs.data(),
-                            tr.csc_vals.data()),
+        check(als_set_train(engine, m, n, (int64_t)tr.val_cnt, tr.h_csr_row_ptrs, tr.h_csr_coo_col_idxs,
+                            tr.h_csr_coo_vals, tr.h_csc_col_ptrs, tr.h_csc_row_idxs, tr.h_csc_vals),
               "als_set_train");
-        check(als_set_test(engine, (int64_t)te.val_cnt, te.coo_row_idxs.data(), te.csr_coo_col_idxs.data(),
-                           te.csr_coo_vals.data()),
+        check(als_set_test(engine, (int64_t)te.val_cnt, te.h_coo_row_idxs, te.h_csr_coo_col_idxs,
+                           te.h_csr_coo_vals),
               "als_set_test");
+        h_UT = static_cast<float *>(std::calloc(std::max<size_t>(1, (size_t)m * f), sizeof(float)));
+        h_VT = static_cast<float *>(std::calloc(std::max<size_t>(1, (size_t)n * f), sizeof(float)));
+        if (!h_UT || !h_VT) throw std::runtime_error("out of host memory for h_UT / h_VT");
     } catch (...) {
+        std::free(h_UT);
+        std::free(h_VT);
+        h_UT = h_VT = nullptr;
         als_destroy(engine);
         engine = nullptr;
         throw;
     }
-    h_UT.assign((size_t)m * f, 0.f);
-    h_VT.assign((size_t)n * f, 0.f);
 }
 
-als_model::~als_model() { als_destroy(engine); }
+als_model::~als_model() {
+    als_destroy(engine);
+    std::free(h_UT);
+    std::free(h_VT);
+}
+
+void als_model::LU_solve_U(int m_batch_size, int m_batch_offset) {
+    check(als_update_rows(engine, ALS_SIDE_USERS, m_batch_offset, m_batch_size, 1), "LU_solve_U");
+}
+
+void als_model::LU_solve_V(int n_batch_size, int n_batch_offset) {
+    check(als_update_rows(engine, ALS_SIDE_ITEMS, n_batch_offset, n_batch_size, 1), "LU_solve_V");
+}
 
 void als_model::train(std::vector<double> *rmse) {
     check(als_init_factors(engine, ALS_INIT_REFERENCE), "als_init_factors");
     if (rmse) rmse->assign(iters, 0.0);
     check(als_train(engine, iters, rmse ? rmse->data() : nullptr), "als_train");
-    check(als_get_factors(engine, ALS_SIDE_USERS, h_UT.data(), m, f), "als_get_factors(U)");
-    check(als_get_factors(engine, ALS_SIDE_ITEMS, h_VT.data(), n, f), "als_get_factors(V)");
+    check(als_get_factors(engine, ALS_SIDE_USERS, h_UT, m, f), "als_get_factors(U)");
+    check(als_get_factors(engine, ALS_SIDE_ITEMS, h_VT, n, f), "als_get_factors(V)");
 }
 
 void als_model::update_U() { check(als_update(engine, ALS_SIDE_USERS), "als_update(U)"); }
@@ -155,4 +186,31 @@ std::string als_model::to_string(SOLVE_TYPE t) {
     }
 }
 
-}  // namespace als_mi355x
+// ---------------------------------------------------- cuMF CG host entries
+namespace {
+
+// The reference's loop `for (int iter = 0; iter < cgIter; iter++)` with a
+// float cgIter runs ceil(cgIter) times (none for cgIter <= 0 or NaN).
+int cg_iterations(float cgIter) {
+    if (!(cgIter > 0.f)) return 0;
+    return cgIter >= 1e9f ? 1000000000 : (int)std::ceil(cgIter);
+}
+
+void exit_on(int rc, const char *what) {
+    if (rc == ALS_OK) return;
+    std::fprintf(stderr, "%s failed (%d): batch, f (1..143) or a HIP error\n", what, rc);
+    std::exit(EXIT_FAILURE);
+}
+
+constexpr float CG_ERROR = 1e-4f;  // cumf_als_cg/cg.cu:32
+
+}  // namespace
+
+void updateXWithCGHost(float *A, float *x, float *b, const int batchSize, const int f, const float cgIter) {
+    exit_on(als_cg_update_x(A, x, b, batchSize, f, cg_iterations(cgIter), CG_ERROR, nullptr), "updateXWithCGHost");
+}
+
+void updateXWithCGHost_tt_fp16(float *A, float *x, float *b, const int batchSize, const int f, const float cgIter) {
+    exit_on(als_cg_update_x_tt_fp16(A, x, b, batchSize, f, cg_iterations(cgIter), CG_ERROR, nullptr),
+            "updateXWithCGHost_tt_fp16");
+}

@@ -43,9 +43,8 @@ static const void *kernel_for(int T) {
 
 template <int T>
 static int lds_for(int solve, int refine) {
-    (void)solve;
     (void)refine;
-    return Geo<T, ALS_PREC>::lds_bytes();
+    return solve == SOLVE_CHOL_REFINE ? Geo<T, ALS_PREC>::refine_lds_bytes() : Geo<T, ALS_PREC>::lds_bytes();
 }
 
 #define ALS_CAT2(a, b) a##b

@@ -168,6 +168,22 @@ class Engine:
         self._check(lib.als_adjoint_dev(self.h, side, C.c_void_p(d_Y), C.c_void_p(d_X), C.c_void_p(d_gbar),
                                         C.c_void_p(d_xbar), C.c_void_p(d_lbar), C.c_void_p(d_ybar or 0)))
 
+    def factor_save_bytes(self, side: int) -> int:
+        v = C.c_int64()
+        self._check(lib.als_factor_save_bytes(self.h, side, C.byref(v)))
+        return v.value
+
+    def forward_dev_ws(self, side: int, d_Y: int, d_X: int, d_lambda: int | None, d_save: int,
+                       save_bytes: int) -> None:
+        self._check(lib.als_forward_dev_ws(self.h, side, C.c_void_p(d_Y), C.c_void_p(d_X), C.c_void_p(d_lambda or 0),
+                                           C.c_void_p(d_save), save_bytes))
+
+    def adjoint_dev_ws(self, side: int, d_Y: int, d_X: int, d_save: int, save_bytes: int, d_gbar: int, d_xbar: int,
+                       d_lbar: int, d_ybar: int | None) -> None:
+        self._check(lib.als_adjoint_dev_ws(self.h, side, C.c_void_p(d_Y), C.c_void_p(d_X), C.c_void_p(d_save),
+                                           save_bytes, C.c_void_p(d_gbar), C.c_void_p(d_xbar), C.c_void_p(d_lbar),
+                                           C.c_void_p(d_ybar or 0)))
+
 
 def cg_update_x(A, x, b, cg_iters: int = 6, tol: float = 1e-4) -> None:
     """Batched CG on device tensors, the reference's updateXWithCGHost

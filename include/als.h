@@ -29,7 +29,9 @@
  *                              als_model.cu:1722-1764
  *   als_get_factors            h_UT / h_VT after train(), als_model.cu:2283-2284
  *   als_save_libmf             als_model::save_libmf, als_model.cu:2294-2312
- *   als_cg_update_x / als_cg_update_x_fp16
+ *   als_update_rows            LU_solve_U / LU_solve_V (one batch of rows),
+ *                              als_model.h:35-36, als_model.cu:1570-1720
+ *   als_cg_update_x / als_cg_update_x_fp16 / als_cg_update_x_tt_fp16
  *                              updateXWithCGHost / updateXWithCGHost_tt_fp16,
  *                              cumf_als_cg/cg.h:30-32 (kernels cg.cu:37-232,
  *                              236-430; host side cg.cu:642-688)
@@ -130,6 +132,14 @@ int als_set_factors(als_engine *e, int side, const float *host, int64_t rows, in
 int als_get_factors(als_engine *e, int side, float *host, int64_t rows, int f);
 
 int als_update(als_engine *e, int side);
+/* Rows [row_begin, row_begin + row_count) of one side only, from the
+ * current factors of the other: one batch of the reference's update blocks
+ * (LU_solve_U / LU_solve_V with their batch size and offset,
+ * als_model.h:35-36, als_model.cu:1570-1720, called per batch at
+ * :1996-2011). exact_solve != 0 solves by Cholesky whatever the configured
+ * solver (the reference's LU_solve_* always factorise); 0 uses it.
+ * Synchronous. */
+int als_update_rows(als_engine *e, int side, int64_t row_begin, int64_t row_count, int exact_solve);
 int als_sweep(als_engine *e);
 /* sweeps < 0 runs cfg.iters; rmse_per_sweep (nullable) receives the test
  * RMSE after every sweep. */
@@ -158,6 +168,11 @@ int als_cg_update_x(const float *A, float *x, const float *b, int batch, int f, 
                     void *hip_stream);
 int als_cg_update_x_fp16(const uint16_t *A, float *x, const float *b, int batch, int f, int cg_iters, float tol,
                          void *hip_stream);
+/* updateXWithCGHost_tt_fp16's own argument types (cumf_als_cg/cg.h:32): an
+ * FP32 A whose entries are rounded to FP16 as they are loaded, then the
+ * same CG (what the reference's kernel intends with its FP16 A). */
+int als_cg_update_x_tt_fp16(const float *A, float *x, const float *b, int batch, int f, int cg_iters, float tol,
+                            void *hip_stream);
 
 /* Host-only (no device needed): the work schedule the engine builds for one
  * side from its CSR row pointers: whole rows plus parts of rows longer than
@@ -218,11 +233,24 @@ int als_device_factors(als_engine *e, int side, float **d_ptr);
  *   als_adjoint_dev: given gbar = dL/dX, writes xbar = A^-1 gbar,
  *     lambda_bar = -n_u xbar_u . x_u, and ybar (the fixed-factor gradient
  *     through A and b; d_ybar rows x ld is overwritten).
- * Requires solve_type LU or CHOL. d_lambda_rows may be NULL (scalar lambda). */
+ * Requires solve_type LU or CHOL. d_lambda_rows may be NULL (scalar lambda).
+ * The engine keeps ONE saved factorisation (the last saving call's);
+ * als_adjoint_dev refuses (ALS_E_STATE) when it belongs to the other side.
+ * Callers that chain several half-sweeps in one differentiated graph keep
+ * one factorisation per call in a buffer of their own (the _ws forms). */
 int als_forward_dev(als_engine *e, int side, const float *d_Y, float *d_X,
                     const float *d_lambda_rows, int save_factor);
 int als_adjoint_dev(als_engine *e, int side, const float *d_Y, const float *d_X,
                     const float *d_gbar, float *d_xbar, float *d_lambda_bar, float *d_ybar);
+/* Caller-owned factorisation: d_save (device, at least
+ * als_factor_save_bytes(side) bytes) receives the forward pass's factor
+ * tiles and is read by the matching adjoint. No reference counterpart. */
+int als_factor_save_bytes(als_engine *e, int side, int64_t *bytes);
+int als_forward_dev_ws(als_engine *e, int side, const float *d_Y, float *d_X,
+                       const float *d_lambda_rows, float *d_save, int64_t save_bytes);
+int als_adjoint_dev_ws(als_engine *e, int side, const float *d_Y, const float *d_X,
+                       const float *d_save, int64_t save_bytes, const float *d_gbar,
+                       float *d_xbar, float *d_lambda_bar, float *d_ybar);
 
 #ifdef __cplusplus
 }

@@ -187,6 +187,51 @@ int orc_cg_solve(int f, const double *A, const double *b, double *x, int iters, 
     return 0;
 }
 
+/* The recurrence of orc_cg_solve in FP32 arithmetic (ORC_SOLVE_CG32): A, b
+ * and x rounded to float, every product, sum and division in float, sums in
+ * index order, the exit test on the float r.r against (float)tol. */
+int orc_cg_solve_f32(int f, const double *A, const double *b, double *x, int iters, double tol) {
+    float *Af = (float *)malloc(sizeof(float) * (size_t)f * f);
+    float *v = (float *)malloc(sizeof(float) * (size_t)f * 4);
+    float *xf = v, *r = v + f, *p = v + 2 * f, *ap = v + 3 * f;
+    const float tolf = (float)tol;
+    for (size_t e = 0; e < (size_t)f * f; e++) Af[e] = (float)A[e];
+    float rsold = 0.f;
+    for (int i = 0; i < f; i++) xf[i] = (float)x[i];
+    for (int i = 0; i < f; i++) {
+        float s = 0.f;
+        for (int j = 0; j < f; j++) s += Af[(size_t)i * f + j] * xf[j];
+        r[i] = (float)b[i] - s;
+        p[i] = r[i];
+        rsold += r[i] * r[i];
+    }
+    for (int it = 0; it < iters; it++) {
+        float pap = 0.f;
+        for (int i = 0; i < f; i++) {
+            float s = 0.f;
+            for (int j = 0; j < f; j++) s += Af[(size_t)i * f + j] * p[j];
+            ap[i] = s;
+            pap += p[i] * s;
+        }
+        if (pap == 0.f) break;
+        const float alpha = rsold / pap;
+        float rsnew = 0.f;
+        for (int i = 0; i < f; i++) {
+            xf[i] += alpha * p[i];
+            r[i] -= alpha * ap[i];
+            rsnew += r[i] * r[i];
+        }
+        if (rsnew < tolf || rsnew == 0.f) break;
+        const float beta = rsnew / rsold;
+        rsold = rsnew;
+        for (int i = 0; i < f; i++) p[i] = r[i] + beta * p[i];
+    }
+    for (int i = 0; i < f; i++) x[i] = (double)xf[i];
+    free(Af);
+    free(v);
+    return 0;
+}
+
 /* One row of the reference's update block (als_model.cu:1891-2011 for users):
  * Gram + RHS, then the configured solver; x holds the warm start for CG. */
 static int update_row(const orc_config *cfg, int64_t n_row, const int *idx,
@@ -199,7 +244,9 @@ static int update_row(const orc_config *cfg, int64_t n_row, const int *idx,
     }
     orc_gram_rhs(f, n_row, idx, val, Y, lambda, cfg->gram_prec, A, b);
     int rc = 0;
-    if (cfg->solver == ORC_SOLVE_CG || cfg->solver == ORC_SOLVE_CG_FP16A) {
+    if (cfg->solver == ORC_SOLVE_CG32) {
+        rc = orc_cg_solve_f32(f, A, b, x, cfg->cg_iters, cfg->cg_tol);
+    } else if (cfg->solver == ORC_SOLVE_CG || cfg->solver == ORC_SOLVE_CG_FP16A) {
         if (cfg->solver == ORC_SOLVE_CG_FP16A)
             for (size_t e = 0; e < (size_t)f * f; e++) A[e] = orc_round(A[e], ORC_PREC_FP16);
         rc = orc_cg_solve(f, A, b, x, cfg->cg_iters, cfg->cg_tol);

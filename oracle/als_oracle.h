@@ -43,7 +43,12 @@ extern "C" {
 /* ORC_SOLVE_CG_FP16A: the CG recurrence on A rounded to FP16 storage (b and
  * the CG vectors stay in full precision): what the reference's
  * CUMF_ALS_CG_FP16 solver intends (cumf_als_cg/cg.cu:236-430). */
-enum { ORC_SOLVE_LU = 0, ORC_SOLVE_CG_FP16A = 1, ORC_SOLVE_CG = 2, ORC_SOLVE_CHOL = 3 };
+/* ORC_SOLVE_CG32: the same CG recurrence carried in FP32 (A, b, x and every
+ * vector, product and sum rounded to float; sums in index order): what an
+ * FP32 execution of the cuMF kernel computes up to its summation order. Its
+ * distance to ORC_SOLVE_CG measures how far FP32 rounding alone moves the
+ * truncated (6-iteration) CG iterate. */
+enum { ORC_SOLVE_LU = 0, ORC_SOLVE_CG_FP16A = 1, ORC_SOLVE_CG = 2, ORC_SOLVE_CHOL = 3, ORC_SOLVE_CG32 = 4 };
 enum { ORC_PREC_EXACT = 0, ORC_PREC_BF16 = 1, ORC_PREC_FP16 = 2, ORC_PREC_FP32 = 3 };
 
 typedef struct {
@@ -71,6 +76,8 @@ int orc_chol_solve(int f, double *A, double *b);          /* b <- A^-1 b */
 int orc_lu_nopiv_solve(int f, double *A, double *b);      /* b <- A^-1 b */
 int orc_cg_solve(int f, const double *A, const double *b, double *x,
                  int iters, double tol);                   /* x in/out */
+int orc_cg_solve_f32(int f, const double *A, const double *b, double *x,
+                     int iters, double tol);               /* x in/out, FP32 recurrence */
 
 /* One half-sweep: update every row of X from the fixed factors Y.
  * lambda_rows (nullable) overrides cfg->lambda per row.

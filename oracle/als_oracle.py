@@ -12,7 +12,7 @@ from pathlib import Path
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libals_oracle.so"
-SOLVE_LU, SOLVE_CG_FP16A, SOLVE_CG, SOLVE_CHOL = 0, 1, 2, 3
+SOLVE_LU, SOLVE_CG_FP16A, SOLVE_CG, SOLVE_CHOL, SOLVE_CG32 = 0, 1, 2, 3, 4
 PREC_EXACT, PREC_BF16, PREC_FP16, PREC_FP32 = 0, 1, 2, 3
 
 
@@ -29,6 +29,7 @@ _SIG = {
     "orc_chol_solve": (C.c_int, [C.c_int, P, P]),
     "orc_lu_nopiv_solve": (C.c_int, [C.c_int, P, P]),
     "orc_cg_solve": (C.c_int, [C.c_int, P, P, P, C.c_int, C.c_double]),
+    "orc_cg_solve_f32": (C.c_int, [C.c_int, P, P, P, C.c_int, C.c_double]),
     "orc_update": (C.c_int64, [C.POINTER(OrcConfig), C.c_int64, P, P, P, P, P, P]),
     "orc_update_rows": (C.c_int64, [C.POINTER(OrcConfig), C.c_int64, P, P, P, P, P, P, P, P]),
     "orc_rmse": (C.c_double, [C.c_int64, P, P, P, P, P, C.c_int]),
@@ -95,9 +96,9 @@ def lu_solve(A, b):
     return rc, b
 
 
-def cg_solve(A, b, x0, iters, tol):
+def cg_solve(A, b, x0, iters, tol, fp32=False):
     A = _c(A, np.float64); b = _c(b, np.float64); x = _c(x0, np.float64).copy()
-    lib.orc_cg_solve(A.shape[0], _p(A), _p(b), _p(x), iters, tol)
+    (lib.orc_cg_solve_f32 if fp32 else lib.orc_cg_solve)(A.shape[0], _p(A), _p(b), _p(x), iters, tol)
     return x
 
 

@@ -31,3 +31,19 @@ def gpu_available() -> bool:
 def toy():
     import cuda_als_amd as ca
     return ca.toy_1k().generate()
+
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def reference_toy(name):
+    """The reference's toy dataset `name` (its prep-log arrays) as RatingData,
+    and the values of its INI file (`section.key` -> str)."""
+    import json
+
+    import cuda_als_amd as ca
+    d = json.loads((GOLDEN / f"{name}.json").read_text())
+    a = d["arrays"]
+    data = ca.from_arrays(d["m"], d["n"], a["R_train_csr.indptr"], a["R_train_csr.indices"], a["R_train_csr.data"],
+                          a["R_test_coo.row"], a["R_test_coo.col"], a["R_test_coo.data"])
+    return data, d["ini"]

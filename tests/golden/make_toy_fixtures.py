@@ -8,8 +8,9 @@ printed arrays (data, not code) into JSON fixtures:
 
     python tests/golden/make_toy_fixtures.py [/root/reference]
 
-and records the matrix sizes the reference's configs use for them
-(config/toy_data.ini:9-12, config/toy_data_2.ini).
+and records the run configuration the reference's configs give for them
+(the key/value pairs of config/toy_data.ini and config/toy_data_2.ini: sizes,
+f, lambda, sweeps, Gram type, solver, runs).
 """
 from __future__ import annotations
 
@@ -45,18 +46,38 @@ def parse_log(text: str) -> dict:
     return out
 
 
+def parse_ini(text: str) -> dict:
+    """`section.key` -> value of an INI file (comments and blank lines dropped)."""
+    out, section = {}, ""
+    for line in text.splitlines():
+        line = line.strip()
+        if not line or line[0] in ";#":
+            continue
+        if line.startswith("["):
+            section = line[1:line.index("]")].strip()
+            continue
+        if "=" in line:
+            k, v = line.split("=", 1)
+            out[f"{section}.{k.strip()}"] = v.strip()
+    return out
+
+
 def main(ref: str) -> None:
     root = Path(ref)
     specs = {
-        "toy_data": (root / "data/toy_data/prepare_toy_data.log", 15, 13),
-        "toy_data_2": (root / "data/toy_data_2/prepare_toy_data_2.log", 32, 23),
+        "toy_data": (root / "data/toy_data/prepare_toy_data.log", root / "config/toy_data.ini", 15, 13),
+        "toy_data_2": (root / "data/toy_data_2/prepare_toy_data_2.log", root / "config/toy_data_2.ini", 32, 23),
     }
-    for name, (log, m, n) in specs.items():
+    for name, (log, ini, m, n) in specs.items():
         arrays = parse_log(log.read_text())
         missing = [k for k in KEYS if k not in arrays]
         if missing:
             raise SystemExit(f"{log}: missing arrays {missing}")
-        fixture = {"source": f"reference {log.relative_to(root)}", "m": m, "n": n, "arrays": arrays}
+        conf = parse_ini(ini.read_text())
+        assert int(conf["data.m"]) == m and int(conf["data.n"]) == n, (ini, conf)
+        # the reference's run configuration for this dataset (the values of its INI file)
+        fixture = {"source": f"reference {log.relative_to(root)}", "m": m, "n": n, "arrays": arrays,
+                   "ini_source": f"reference {ini.relative_to(root)}", "ini": conf}
         (HERE / f"{name}.json").write_text(json.dumps(fixture, indent=1))
         print("wrote", HERE / f"{name}.json")
 

@@ -101,3 +101,64 @@ def test_driver_reports_missing_files(tmp_path, toy):
     r = subprocess.run([str(MAIN), str(ini)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "error" in r.stderr
+
+
+@pytest.mark.parametrize("name", ["toy_data", "toy_data_2"])
+def test_reference_toy_configs_match_oracle(tmp_path, name):
+    """The reference's own runnable configurations (config/toy_data.ini:
+    15 x 13, f = 100; config/toy_data_2.ini: 32 x 23, f = 20; both SIMPLE
+    Gram, LU solver, lambda 0.05, 10 sweeps, 10 runs) on their prepared
+    arrays (the prep-log golden fixtures), through the driver: the test RMSE
+    of every sweep (the CALC_ITER_RMSE lines and the final one) of every run
+    within 1e-4 of the FP64 oracle. Factors after 10 sweeps: with one
+    FP64-residual refinement step ([als] refine_steps = 1) within 1e-4; as
+    configured (plain FP32 solve) within 1.5x the distance the same ALS done
+    in float32 by numpy/LAPACK has from the oracle (1.3e-4 and 1.4e-4 on
+    these two: on these heavily underdetermined rows the sweeps amplify
+    FP32 rounding), and 1e-4 where that is larger."""
+    from oracle import als_oracle as O
+
+    from .conftest import reference_toy
+    d, ini = reference_toy(name)
+    data, log = tmp_path / "data", tmp_path / "log"
+    data.mkdir()
+    log.mkdir()
+    ca.save_bin(d, str(data))
+    f, lam, iters, runs = (int(ini["als.f"]), float(ini["als.lambda"]), int(ini["als.als_iters"]),
+                           int(ini["general.als_runs"]))
+    U0, V0 = O.reference_init(d.m, d.n, f)
+    Uo, Vo, ro, bad = O.train(O.config(f, lam, O.SOLVE_CHOL, store_fp32=True), d, iters, U0, V0)
+    assert bad == 0
+
+    def run(conf, label):
+        sections = {}
+        for k, v in conf.items():
+            sec, key = k.split(".", 1)
+            sections.setdefault(sec, []).append(f"{key} = {v}")
+        text = "".join(f"[{s}]\n" + "\n".join(lines) + "\n\n" for s, lines in sections.items())
+        p = tmp_path / f"{label}.ini"
+        p.write_text(text)
+        r = subprocess.run([str(MAIN), str(p)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return r.stdout
+
+    conf = dict(ini, **{"general.log_folder": str(log), "data.data_folder": str(data)})
+    out = run(conf, "ref")
+    per_sweep = [float(x) for x in re.findall(r"RMSE test: ([0-9.]+)", out)]
+    assert len(per_sweep) == runs * iters     # iters - 1 per-sweep lines + the final line, per run
+    got = np.array(per_sweep).reshape(runs, iters)
+    print(name, "driver", got[0], "\noracle", ro)
+    assert np.abs(got - ro[None, :]).max() <= 1e-4
+    # single runs with the model dump: the factors
+    from .test_gpu_parity import lapack_fp32_train
+    Ul, Vl, _ = lapack_fp32_train(d, f, iters, lam)
+    lapack = max(np.abs(Ul - Uo).max(), np.abs(Vl - Vo).max())
+    for refine, bound in (("0", max(1e-4, 1.5 * lapack)), ("1", 1e-4)):
+        model = tmp_path / f"model{refine}.bin"
+        run(dict(conf, **{"general.als_runs": "1", "general.model_path": str(model), "als.refine_steps": refine}),
+            f"dump{refine}")
+        body = np.frombuffer(model.read_bytes()[20:], np.float32)
+        Um, Vm = body[:d.m * f].reshape(d.m, f), body[d.m * f:].reshape(d.n, f)
+        err = max(np.abs(Um - Uo).max(), np.abs(Vm - Vo).max())
+        print(f"refine_steps {refine}: factor err {err:.3e} (float32 LAPACK: {lapack:.3e})")
+        assert err <= bound

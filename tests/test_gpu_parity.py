@@ -3,10 +3,12 @@ identical seeded inputs.
 
 Tolerances (absolute):
   FP32 path: per-sweep test RMSE and factors within 1e-4 of the oracle.
-  Mixed precision (16-bit Gram, FP32 Cholesky + FP64-residual refinement):
-    per-sweep test RMSE within 2e-4 of the exact FP64 oracle (FP16 Gram);
-    BF16 Gram: within 1e-4 of the oracle run on BF16-rounded factors (its
-    distance to the exact oracle is the format's own error and is reported).
+  Mixed precision (FP16 or BF16 Gram, FP32 Cholesky + FP64-residual
+    refinement restaged from the FP32 master rows): per-sweep test RMSE
+    within 2e-4 of the exact FP64 oracle.
+  CG-6: per half-sweep within 1e-4 without the early exit; trajectories
+    within twice the distance the oracle's own FP32 execution of the
+    recurrence has from its FP64 one.
 """
 import numpy as np
 import pytest
@@ -47,20 +49,58 @@ def test_reference_init_on_device(toy):
     assert not U.any()
 
 
+def lapack_fp32_train(d, f, sweeps, lam=0.05):
+    """Reference point for FP32 drift: the same ALS with each normal
+    equation formed and solved in float32 by numpy/LAPACK (sgemm Gram,
+    spotrf/spotrs), factors stored as float32."""
+    import scipy.linalg as sl
+    U0, V0 = O.reference_init(d.m, d.n, f)
+    U, V = U0.astype(np.float32), V0.astype(np.float32)
+
+    def half(ptr, idx, val, Y, X):
+        out = np.zeros_like(X)
+        for u in range(len(ptr) - 1):
+            s, e = ptr[u], ptr[u + 1]
+            if e == s:
+                continue
+            Yu = Y[idx[s:e]]
+            A = Yu.T @ Yu + np.float32(lam * (e - s)) * np.eye(f, dtype=np.float32)
+            out[u] = sl.cho_solve(sl.cho_factor(A), Yu.T @ val[s:e])
+        return out
+
+    rm = []
+    for _ in range(sweeps):
+        U = half(d.csr_ptr, d.csr_idx, d.csr_val, V, U)
+        V = half(d.csc_ptr, d.csc_idx, d.csc_val, U, V)
+        rm.append(O.rmse(d.t_row, d.t_col, d.t_val, U, V))
+    return U, V, np.array(rm)
+
+
 @pytest.mark.parametrize("solve", [E.SolveType.CHOL, E.SolveType.LU])
-def test_fp32_cholesky_matches_oracle(toy, solve):
-    """Per-sweep test RMSE within 1e-4 of the FP64 oracle over 5 sweeps.
-    Factors: within 1e-4 after the first sweep; each FP32 half-sweep adds
-    ~4e-6 (FP32 Gram accumulation, see the next test) and over 10 half-sweeps
-    the drift reaches ~1.1e-4 on this workload, bounded here at 2e-4."""
-    U, V, r, info = run_gpu(toy, 16, solve, E.Prec.FP32, 5)
+@pytest.mark.parametrize("refine", [0, 1])
+def test_fp32_cholesky_matches_oracle(toy, solve, refine):
+    """Per-sweep test RMSE within 1e-4 of the FP64 oracle over 5 sweeps, and
+    factors within 1e-4 after the first sweep. After 5 sweeps: with one
+    FP64-residual refinement step (restaged from the ratings) the factors
+    are within 1e-4 too. The plain FP32 solve drifts further on this
+    workload's ill-conditioned few-rating items (each half-sweep adds FP32
+    Gram accumulation and factorisation error, kappa * eps); its drift is
+    bounded by 1.5x that of the same ALS done in float32 with numpy/LAPACK
+    (sgemm + spotrf), printed beside it."""
+    U, V, r, info = run_gpu(toy, 16, solve, E.Prec.FP32, 5, refine=refine)
     Uo, Vo, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
-    print("gpu", r, "\noracle", ro, "\nfactor drift", np.abs(U - Uo).max(), np.abs(V - Vo).max())
+    drift = max(np.abs(U - Uo).max(), np.abs(V - Vo).max())
+    print("gpu", r, "\noracle", ro, "\nfactor drift", drift)
     assert info.failures == 0
     assert np.max(np.abs(r - ro)) <= 1e-4
-    assert np.max(np.abs(U - Uo)) <= 2e-4
-    assert np.max(np.abs(V - Vo)) <= 2e-4
-    U1, V1, _, _ = run_gpu(toy, 16, solve, E.Prec.FP32, 1)
+    if refine:
+        assert drift <= 1e-4
+    else:
+        Ul, Vl, rl = lapack_fp32_train(toy, 16, 5)
+        lapack_drift = max(np.abs(Ul - Uo).max(), np.abs(Vl - Vo).max())
+        print("float32 LAPACK drift", lapack_drift, "rmse diff", np.abs(rl - ro).max())
+        assert drift <= max(1e-4, 1.5 * lapack_drift)
+    U1, V1, _, _ = run_gpu(toy, 16, solve, E.Prec.FP32, 1, refine=refine)
     Uo1, Vo1, _ = run_oracle(toy, 16, O.SOLVE_CHOL, 1)
     assert np.max(np.abs(U1 - Uo1)) <= 1e-4 and np.max(np.abs(V1 - Vo1)) <= 1e-4
 
@@ -83,40 +123,57 @@ def test_fp32_half_sweep_error_from_trained_state(toy, f, refine):
     assert err <= 1e-5
 
 
-def test_fp16_mixed_matches_exact_oracle(toy):
-    U, V, r, info = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.FP16, 5, refine=2)
-    _, _, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
-    _, _, re = run_oracle(toy, 16, O.SOLVE_CHOL, 5, prec=O.PREC_FP16)
-    print("gpu", r, "\nexact", ro, "\nfp16-rounded oracle", re)
+def run_gpu_restaged(d, f, prec, sweeps, refine):
+    """Classic refinement with every row's residual restaged from the FP32
+    master rows (the residual of the unrounded system)."""
+    cfg = E.make_config(f, 0.05, sweeps, E.SolveType.CHOL, prec, refine=refine)
+    with E.Engine(cfg, d) as e:
+        for side in (0, 1):
+            e.set_schedule(side, restage_max=1 << 30)
+        e.init_factors()
+        r = e.train(sweeps)
+        return e.get_factors(0), e.get_factors(1), r, e.info()
+
+
+@pytest.mark.parametrize("prec,oprec", [(E.Prec.FP16, O.PREC_FP16), (E.Prec.BF16, O.PREC_BF16)])
+def test_mixed_trajectory_matches_exact_oracle(toy, prec, oprec):
+    """16-bit Gram (FP16 or BF16), FP32 Cholesky, two refinement steps whose
+    FP64 residual is formed from the FP32 master rows: the correction solves
+    on the 16-bit factor converge to the unrounded system, so the test RMSE
+    of every sweep is within the mixed-precision bar (2e-4) of the exact FP64
+    oracle, also for BF16, whose rounded system is itself further away (the
+    FP64 oracle fed BF16-rounded factors, printed, moves by ~2e-4 here)."""
+    U, V, r, info = run_gpu_restaged(toy, 16, prec, 5, refine=2)
+    Uo, Vo, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
+    _, _, re = run_oracle(toy, 16, O.SOLVE_CHOL, 5, prec=oprec)
+    print("gpu", r, "\nexact", ro, "\n16-bit-rounded oracle", re, "\ngpu - exact", np.abs(r - ro).max(),
+          "rounded - exact", np.abs(re - ro).max(), "factors", np.abs(U - Uo).max(), np.abs(V - Vo).max())
+    assert info.failures == 0
     assert np.max(np.abs(r - ro)) <= 2e-4
-    assert np.max(np.abs(r - re)) <= 1e-4
 
 
-def test_bf16_half_sweep_matches_rounded_oracle(toy):
-    """Same BF16-rounded inputs on both sides: the kernel must agree with the
-    FP64 solve of the BF16-formed system to FP32 accuracy."""
+def test_bf16_half_sweep(toy):
+    """One users half-sweep from the reference init with a BF16 Gram. Plain
+    FP32 Cholesky: the FP64 solve of the BF16-formed system to FP32 accuracy.
+    Two refinement steps restaged from the FP32 master rows: the exact
+    system's solve (printed: how far the BF16-formed system's solve is)."""
     f = 16
     U0, V0 = O.reference_init(toy.m, toy.n, f)
-    with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.BF16, refine=2), toy) as e:
-        e.init_factors()
-        e.update(ca.ALS_SIDE_USERS)
-        U = e.get_factors(0)
-    Uo, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=O.PREC_BF16), toy.csr_ptr, toy.csr_idx,
+    out = {}
+    for refine in (0, 2):
+        with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.BF16, refine=refine), toy) as e:
+            e.set_schedule(0, restage_max=1 << 30)
+            e.init_factors()
+            e.update(ca.ALS_SIDE_USERS)
+            out[refine] = e.get_factors(0)
+    Ub, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=O.PREC_BF16), toy.csr_ptr, toy.csr_idx,
                      toy.csr_val, V0, U0)
-    assert np.max(np.abs(U - Uo)) <= 1e-5 * max(1.0, np.abs(Uo).max())
-
-
-def test_bf16_mixed_trajectory_error_is_reported(toy):
-    """Over sweeps, BF16 rounding flips amplify; the trajectory's distance to
-    the exact oracle is the format's error (the FP64 oracle fed BF16-rounded
-    factors is itself 2.2e-4 away on this workload). Bound it at 5e-4."""
-    U, V, r, info = run_gpu(toy, 16, E.SolveType.CHOL, E.Prec.BF16, 5, refine=2)
-    _, _, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
-    _, _, re = run_oracle(toy, 16, O.SOLVE_CHOL, 5, prec=O.PREC_BF16)
-    print("gpu", r, "\nexact", ro, "\nbf16-rounded oracle", re, "\nbf16 error vs exact", np.abs(r - ro).max())
-    assert info.failures == 0
-    assert np.max(np.abs(r[:3] - re[:3])) <= 1e-5
-    assert np.max(np.abs(r - ro)) <= 5e-4
+    Ux, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL), toy.csr_ptr, toy.csr_idx, toy.csr_val, V0, U0)
+    scale = max(1.0, np.abs(Ux).max())
+    print("plain vs bf16 system", np.abs(out[0] - Ub).max(), "refined vs exact", np.abs(out[2] - Ux).max(),
+          "bf16 system vs exact", np.abs(Ub - Ux).max())
+    assert np.max(np.abs(out[0] - Ub)) <= 1e-5 * scale
+    assert np.max(np.abs(out[2] - Ux)) <= 1e-5 * scale
 
 
 def _cg_half_sweep(d, f, solve, osolve, tol, iters=6):
@@ -143,28 +200,47 @@ def test_cg_half_sweep_matches_oracle_without_early_exit(toy, solve, osolve):
     assert err <= 1e-4
 
 
-def test_cg_early_exit_flips_are_rare(toy):
-    """With tol = 1e-4 a row whose r.r lands near the threshold can stop one
-    iteration earlier in FP32 than in FP64: only those rows may differ."""
+def test_cg_early_exit_flips_are_isolated(toy):
+    """With tol = 1e-4 a row whose r.r lands near the threshold can take one
+    iteration fewer or more in FP32 than in FP64. Those rows are found from
+    the FP64 oracle alone: the ones whose result changes when tol moves by
+    +-5% (their exit decision depends on r.r within 5% of tol). Every row
+    where the GPU differs from the FP64 recurrence by more than 1e-4
+    (relative to max(1, |x|)) must be one of them, and they are few."""
     U, Uo = _cg_half_sweep(toy, 16, E.SolveType.CUMF_ALS_CG_FP32, O.SOLVE_CG, 1e-4)
-    diff = np.max(np.abs(U - Uo), axis=1)
-    frac = np.mean(diff > 1e-4)
-    print("rows differing > 1e-4:", frac)
-    assert frac <= 0.02
+    _, lo = _cg_half_sweep(toy, 16, E.SolveType.CUMF_ALS_CG_FP32, O.SOLVE_CG, 0.95e-4)
+    _, hi = _cg_half_sweep(toy, 16, E.SolveType.CUMF_ALS_CG_FP32, O.SOLVE_CG, 1.05e-4)
+    near = (np.abs(lo - Uo).max(axis=1) > 0) | (np.abs(hi - Uo).max(axis=1) > 0)
+    diff = np.abs(U - Uo).max(axis=1) / np.maximum(1.0, np.abs(Uo).max(axis=1))
+    off = diff > 1e-4
+    print("rows near the threshold:", int(near.sum()), "rows off by > 1e-4:", int(off.sum()),
+          "largest off-threshold difference:", float(diff[~near].max()))
+    assert not np.any(off & ~near)
+    assert near.mean() <= 0.05
 
 
 def test_cg_fp32_trajectory(toy):
+    """Five sweeps of CG-6 in FP32 against the FP64 recurrence. The truncated
+    CG iterate is sensitive to rounding: the oracle's own FP32 execution of
+    the same recurrence (ORC_SOLVE_CG32, sums in index order) is ~1e-3 away
+    from the FP64 one in test RMSE on this workload. The GPU's distance is
+    bounded by twice that spread (and 1e-4 when the spread is smaller)."""
     U, V, r, info = run_gpu(toy, 16, E.SolveType.CUMF_ALS_CG_FP32, E.Prec.FP32, 5)
     _, _, ro = run_oracle(toy, 16, O.SOLVE_CG, 5)
-    print("gpu", r, "\noracle", ro)
+    _, _, r32 = run_oracle(toy, 16, O.SOLVE_CG32, 5)
+    spread = np.abs(r32 - ro).max()
+    print("gpu", r, "\noracle fp64", ro, "\noracle fp32", r32, "\ngpu - fp64", np.abs(r - ro).max(),
+          "fp32 oracle - fp64", spread)
     assert info.failures == 0
-    assert np.max(np.abs(r - ro)) <= 2e-3
+    assert np.max(np.abs(r - ro)) <= max(1e-4, 2.0 * spread)
 
 
 @pytest.mark.parametrize("f", [1, 7, 15, 16, 31, 48, 64, 100, 128])
 @pytest.mark.parametrize("prec", [E.Prec.FP32, E.Prec.FP16])
 def test_half_sweep_every_tile_count(toy, f, prec):
-    """One user half-sweep from the reference init, every 16-block count."""
+    """One user half-sweep from the reference init, every 16-block count.
+    FP16 Gram with one classic refinement step, whose residual is restaged
+    from the FP32 master rows: it lands on the exact system's solution."""
     U0, V0 = O.reference_init(toy.m, toy.n, f)
     cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=1 if prec != E.Prec.FP32 else 0)
     with E.Engine(cfg, toy) as e:
@@ -172,8 +248,7 @@ def test_half_sweep_every_tile_count(toy, f, prec):
         e.update(ca.ALS_SIDE_USERS)
         U = e.get_factors(0)
         assert e.info().failures == 0
-    oprec = O.PREC_EXACT if prec == E.Prec.FP32 else O.PREC_FP16
-    Uo, bad = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec), toy.csr_ptr, toy.csr_idx, toy.csr_val,
+    Uo, bad = O.update(O.config(f, 0.05, O.SOLVE_CHOL), toy.csr_ptr, toy.csr_idx, toy.csr_val,
                        V0, U0)
     err = np.max(np.abs(U - Uo)) / max(1.0, np.max(np.abs(Uo)))
     print(f, prec, err)
@@ -295,6 +370,9 @@ def test_split_rows_match_whole_rows_fp16_half_sweep(toy, side):
     for split_len in (8, 1 << 30):
         cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.FP16, refine=1, split_len=split_len)
         with E.Engine(cfg, toy) as e:
+            # every row refines against its FP32-accumulated tiles (split rows
+            # always do), so the two differ only in the order of the sums
+            e.set_schedule(side, restage_max=0)
             e.set_factors(0, U0)
             e.set_factors(1, V0)
             e.update(side)
@@ -333,18 +411,22 @@ def test_autotune_keeps_state_and_results():
             assert r["best_ms"][sd] <= r["default_ms"][sd]
             assert r["split_len"][sd] >= 256 and r["batch_work"][sd] >= 1
         assert list(e1.info().split_len) == r["split_len"]
+        # The tuned restage threshold moves rows between the two refinement
+        # residuals, which converge to different systems (restaged from the
+        # FP32 rows: the exact one; the FP32-accumulated 16-bit tiles: the
+        # FP16-rounded one, ~5e-4 away on the lightest rows here). For the
+        # comparison both engines keep the default threshold, 32 (T + 1).
+        for sd in (0, 1):
+            e1.set_schedule(sd, restage_max=32 * (e1.info().T + 1))
         e0.sweep()
         e0.sweep()
         e1.sweep()
         U0, V0, U1, V1 = e0.get_factors(0), e0.get_factors(1), e1.get_factors(0), e1.get_factors(1)
         assert e1.info().failures == 0
-        # A different split length cuts different parts, and the tuned restage
-        # threshold moves rows between the two refinement residuals (restaged
-        # ratings: the exact FP16-input system; FP32 tile copy: its
-        # accumulation error times the row's conditioning, ~1e-4 on the
-        # lightest rows here).
-        assert np.max(np.abs(U0 - U1)) <= 2e-4 * max(1.0, np.abs(U0).max())
-        assert np.max(np.abs(V0 - V1)) <= 2e-4 * max(1.0, np.abs(V0).max())
+        # a different split length cuts different parts: FP32 summation order
+        print("tuned vs default", np.abs(U0 - U1).max(), np.abs(V0 - V1).max())
+        assert np.max(np.abs(U0 - U1)) <= 2e-5 * max(1.0, np.abs(U0).max())
+        assert np.max(np.abs(V0 - V1)) <= 2e-5 * max(1.0, np.abs(V0).max())
 
 
 @pytest.mark.parametrize("prec,refine,warm", [(E.Prec.BF16, 0, False), (E.Prec.FP16, 0, False),
@@ -408,9 +490,11 @@ def test_group_mixed_models_and_rejections(toy):
 def test_warm_started_refinement(toy):
     """refine_warm: the first refinement step starts from the row's previous
     value x0, with the FP64 residual b - A x0 formed before the
-    factorisation. Its error is kappa * eps * |x* - x0|: from a trained state
-    a half-sweep is as close to the FP64 solve of the FP16-rounded system as
-    classic refinement, and the trajectory meets the mixed-precision bar."""
+    factorisation from the 16-bit Gram. Its error is kappa * eps * |x* - x0|:
+    from a trained state a half-sweep lands on the FP64 solve of the
+    FP16-rounded system; a classic step after it (its residual restaged from
+    the FP32 master rows), or a classic step alone, lands on the exact
+    system's solve. The trajectory meets the mixed-precision bar."""
     f = 16
     U, V, r, info = None, None, None, None
     with E.Engine(E.make_config(f, 0.05, 5, E.SolveType.CHOL, E.Prec.FP16, refine=1, refine_warm=True), toy) as e:
@@ -432,7 +516,8 @@ def test_warm_started_refinement(toy):
             e.set_factors(1, Vo.astype(np.float32))
             e.update(ca.ALS_SIDE_USERS)
             Ug = e.get_factors(0)
-        Un, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=O.PREC_FP16), toy.csr_ptr, toy.csr_idx,
+        oprec = O.PREC_FP16 if (warm, steps) == (True, 1) else O.PREC_EXACT
+        Un, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec), toy.csr_ptr, toy.csr_idx,
                          toy.csr_val, Vo.astype(np.float32), Uo)
         errs[(warm, steps)] = np.abs(Ug - Un).max() / max(1.0, np.abs(Un).max())
     print(errs)

@@ -154,6 +154,20 @@ def test_library_exports_every_declared_symbol():
         getattr(A.lib, n)
 
 
+def test_library_exports_reference_cpp_entry_points():
+    """The cuMF CG host functions with the reference's C++ signatures
+    (cumf_als_cg/cg.h:30,32, declared in include/als_cg.h) and the C++ mirror
+    of als_model / host_sparse_matrix (include/als_model.hpp)."""
+    out = subprocess.run(["nm", "-DC", "--defined-only", str(A.LIB_PATH)], capture_output=True, text=True).stdout
+    for sig in ("updateXWithCGHost(float*, float*, float*, int, int, float)",
+                "updateXWithCGHost_tt_fp16(float*, float*, float*, int, int, float)",
+                "als_model::LU_solve_U(int, int)", "als_model::LU_solve_V(int, int)", "als_model::train(",
+                "als_model::rsme_test()", "als_model::rsme_train()", "als_model::save_libmf(char const*)",
+                "host_sparse_matrix::load_csr(", "host_sparse_matrix::load_csc(", "host_sparse_matrix::load_coo(",
+                "host_sparse_matrix::malloc_load_file("):
+        assert sig in out, sig
+
+
 def test_engine_creation_fails_cleanly_without_a_device():
     from cuda_als_amd import engine as E
     from tests.conftest import gpu_available

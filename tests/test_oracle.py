@@ -128,6 +128,31 @@ def test_cg_warm_start_at_solution_stays_finite():
     assert np.all(np.isfinite(out)) and np.allclose(out, x)
 
 
+def test_cg32_is_the_fp32_execution_of_the_recurrence():
+    """ORC_SOLVE_CG32: every iterate is float32; with enough iterations it
+    solves to FP32 accuracy (numpy's solve); the first step equals the
+    closed form x0 + (r.r / r.Ar) r to FP32 rounding; a 1 x 1 system
+    converges in one step; on a well-conditioned system it stays within a
+    few float32 ulps of the FP64 recurrence after each of 1..4 iterations."""
+    rng = np.random.default_rng(8)
+    M = rng.standard_normal((12, 12))
+    A = M @ M.T + 12 * np.eye(12)
+    b = rng.standard_normal(12)
+    x = O.cg_solve(A, b, np.zeros(12), 40, 0.0, fp32=True)
+    assert np.array_equal(x, x.astype(np.float32).astype(np.float64))
+    want = np.linalg.solve(A, b)
+    assert 0 < np.abs(x - want).max() <= 2e-5 * np.abs(want).max()
+    x0 = rng.standard_normal(12)
+    r0 = b - A @ x0
+    one = x0 + (r0 @ r0) / (r0 @ A @ r0) * r0
+    assert np.abs(O.cg_solve(A, b, x0, 6, 1e30, fp32=True) - one).max() <= 1e-5 * np.abs(one).max()
+    assert abs(O.cg_solve(np.array([[4.0]]), np.array([2.0]), np.array([7.0]), 1, 0.0, fp32=True)[0] - 0.5) < 1e-7
+    for it in range(1, 5):
+        x64 = O.cg_solve(A, b, x0, it, 0.0)
+        x32 = O.cg_solve(A, b, x0, it, 0.0, fp32=True)
+        assert np.abs(x32 - x64).max() <= 1e-5 * np.abs(x64).max(), it
+
+
 # ------------------------------------------------------------- half-sweep
 def _tiny_data(seed=7, m=30, n=20, density=0.3, empty_row=None):
     import cuda_als_amd as ca
