@@ -111,6 +111,8 @@ SIGNATURES = {
     "als_cg_update_x": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
     "als_cg_update_x_fp16": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
     "als_cg_update_x_tt_fp16": (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
+    "als_rmse_dev_noalloc": (C.c_int, [P, P, P, C.c_int64, P, P, C.c_int, C.c_int, P, P]),
+    "als_noalloc_last_error": (C.c_char_p, []),
     # als_data.h
     "als_synth_generate": (C.c_int, [C.POINTER(SynthSpec), C.POINTER(P)]),
     "als_dataset_from_csr": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, P, P, P, C.c_int64, P, P, P, C.POINTER(P)]),

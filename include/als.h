@@ -252,6 +252,25 @@ int als_adjoint_dev_ws(als_engine *e, int side, const float *d_Y, const float *d
                        const float *d_save, int64_t save_bytes, const float *d_gbar,
                        float *d_xbar, float *d_lambda_bar, float *d_ybar);
 
+/* RMSE over caller device buffers with no allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops). The kernels of als_rmse (FP32
+ * dot, FP64 sum, fixed-order reduction, so the same inputs give the same
+ * bits) on a COO of n ratings (d_row, d_col, d_val) and factor rows of
+ * stride ld floats (16-byte aligned, ld a multiple of 4, f <= ld; columns
+ * f.. are ignored): d_out[0] = sum of squared errors, d_out[1] =
+ * sqrt(sum / n) (device memory, 2 doubles). The per-block partial sums live
+ * in a fixed array of the library on each GPU (1024 doubles, 8 KiB, held
+ * until the process ends): n may be anything up to 2^31 - 1 (the engine's
+ * rating limit; larger is refused, ALS_E_INVALID). Calls that overlap on one
+ * GPU race on that array; order them on one stream. Enqueued on hip_stream
+ * (NULL: the default stream); returns without waiting.
+ * als_noalloc_last_error() gives the text of this thread's last failure. */
+int als_rmse_dev_noalloc(const int *d_row, const int *d_col, const float *d_val, int64_t n,
+                         const float *d_U, const float *d_V, int f, int ld, double *d_out,
+                         void *hip_stream);
+const char *als_noalloc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif
