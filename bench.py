@@ -110,6 +110,52 @@ def time_path(torch, E, ca, d, cfg, steps, warmup, tag, autotune=True, oracle=No
     return out
 
 
+def group_leg(torch, E, ca, steps, warmup, autotune=True):
+    """Three models (f = 32, 64, 128; BF16 Gram, FP32 Cholesky + one warm-started
+    refinement step) over the ml10M-shaped workload, updated as one group (one
+    graph launch per half-sweep), against the same three engines run one after
+    the other on one stream and run concurrently on their own streams."""
+    d = ca.ml10m_shape().generate()
+    fs = (32, 64, 128)
+    s0 = torch.cuda.Stream()
+    torch.cuda.set_stream(s0)
+    engs = [E.Engine(E.make_config(f, 0.05, steps, E.SolveType.CHOL, E.Prec.BF16, refine=1, refine_warm=True), d)
+            for f in fs]
+    for e in engs:
+        e.init_factors()
+        if autotune:
+            e.autotune()
+    own = [torch.cuda.Stream() for _ in engs]
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return steps / (time.perf_counter() - t0)
+
+    out = {"workload": f"ml10M-shape {d.m}x{d.n}, {d.nnz} train ratings; models f={list(fs)}, bf16 Gram, "
+                       "fp32 Cholesky + 1 warm-started fp64-residual refinement step", "unit": "sweeps/s (all models)"}
+    for e in engs:
+        e.set_stream(s0.cuda_stream)
+    with E.Group(engs) as g:
+        out["group"] = timed(g.sweep)
+    out["separate_one_stream"] = timed(lambda: [e.sweep() for e in engs])
+    for e, st in zip(engs, own):
+        e.set_stream(st.cuda_stream)
+    out["separate_own_streams"] = timed(lambda: [e.sweep() for e in engs])
+    out["per_model_alone"] = {}
+    for e, f, st in zip(engs, fs, own):
+        out["per_model_alone"][f"f{f}"] = timed(e.sweep)
+    for e in engs:
+        e.close()
+    log(f"[group] {out}")
+    return out
+
+
 def oracle_fixture(d, f):
     """The FP64 oracle's 10-sweep trajectory of this workload
     (tests/golden/netflix_trajectory.json), if it was made from the same
@@ -215,6 +261,7 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost-model schedule")
+    ap.add_argument("--no-group", action="store_true", help="skip the three-model group leg (ml10M shape)")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (debug only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -323,6 +370,8 @@ def main() -> int:
         "test_rmse_delta_vs_oracle": ({"mixed": M["vs_oracle"], "fp32": fp32 and fp32["vs_oracle"],
                                        "fp32_cg6": cg6 and cg6["vs_oracle"]} if fix else None),
         "roofline": roofline,
+        "multi_model_group": None if a.no_group else group_leg(torch, E, ca, a.steps, a.warmup,
+                                                                not a.no_autotune),
         "cpu_baseline": None if a.no_cpu else cpu_baseline(d, f, lam),
     }
     print(json.dumps(line), flush=True)

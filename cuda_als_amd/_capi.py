@@ -44,7 +44,8 @@ class AlsInfo(C.Structure):
     _fields_ = [("T", C.c_int), ("ld", C.c_int), ("grid", C.c_int * 2), ("lds_bytes", C.c_int),
                 ("nbatches", C.c_int * 2), ("nunits", C.c_int * 2), ("nsplit_rows", C.c_int * 2),
                 ("split_len", C.c_int64 * 2), ("failures", C.c_int64), ("m", C.c_int64), ("n", C.c_int64),
-                ("nnz", C.c_int64), ("n_test", C.c_int64), ("dual_max", C.c_int), ("dual_rows", C.c_int * 2)]
+                ("nnz", C.c_int64), ("n_test", C.c_int64), ("dual_max", C.c_int), ("dual_rows", C.c_int * 2),
+                ("kernel_variant", C.c_int * 2), ("grid_variant1", C.c_int)]
 
 
 class ScheduleStats(C.Structure):
@@ -54,7 +55,8 @@ class ScheduleStats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("split_len", C.c_int64 * 2), ("batch_work", C.c_int64 * 2), ("default_ms", C.c_float * 2),
-                ("best_ms", C.c_float * 2), ("candidates", C.c_int * 2), ("restage_max", C.c_int * 2)]
+                ("best_ms", C.c_float * 2), ("candidates", C.c_int * 2), ("restage_max", C.c_int * 2),
+                ("kernel_variant", C.c_int * 2)]
 
 
 class SynthSpec(C.Structure):
@@ -103,6 +105,7 @@ SIGNATURES = {
                                        C.POINTER(ScheduleStats), P, C.c_int64]),
     "als_autotune": (C.c_int, [P, C.c_int, C.POINTER(TuneResult)]),
     "als_set_schedule": (C.c_int, [P, C.c_int, C.c_int64, C.c_int64, C.c_int]),
+    "als_set_kernel_variant": (C.c_int, [P, C.c_int, C.c_int]),
     "als_group_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
     "als_group_destroy": (None, [P]),
     "als_group_last_error": (C.c_char_p, [P]),

@@ -234,7 +234,7 @@ __device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int
 // G[16J + 4q + r][16I + c] = G[16I + c][16J + 4q + r] (I > J), the layout
 // the register Cholesky works in.
 // 16-bit Gram of one row when one gather task per lane covers a chunk
-// (4 * nseg <= 64, f <= 127), software-pipelined: the gathers of chunk k+1
+// (4 * nseg <= 64, f <= 128), software-pipelined: the gathers of chunk k+1
 // are issued before the MFMAs of chunk k and land in registers while they
 // run, and the indices and ratings of chunk k+2 are loaded meanwhile. The
 // stage, the task layout and the MFMA order are those of the plain loop,
@@ -348,10 +348,13 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
 #pragma unroll
     for (int t = 0; t < G::NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     stage_clear<T, E>(a.f, stage);
-    if constexpr (E != PREC_FP32 && T <= 8) {
-        // f <= 16 T - 1 <= 127: at most 16 segments, one task per lane
-        gram_row16_piped<T, E>(a, rs, n, acc, lds);
-        return;
+    if constexpr (E != PREC_FP32 && T <= 9) {
+        // f <= 128: at most 16 segments, one task per lane (every f of
+        // T <= 8; f = 128 of T = 9, whose rating column is the 129th)
+        if (T <= 8 || a.f <= 128) {
+            gram_row16_piped<T, E>(a, rs, n, acc, lds);
+            return;
+        }
     }
     for (int k0 = 0; k0 < n; k0 += CHUNK) {
         stage_chunk<T, E>(a, rs, k0, min(CHUNK, n - k0), lds);

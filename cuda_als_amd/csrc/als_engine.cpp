@@ -34,7 +34,8 @@ struct Side {
     float *partial = nullptr;
     int nbatches = 0, nunits = 0, nsplits = 0;
     int64_t nslots = 0, split_len = 0, batch_work = 0;
-    int grid = 0;
+    int grid = 0;                  // persistent waves of the default update kernel (variant 0)
+    int variant = 0;               // update kernel variant (1: one wave per SIMD, where built)
     int restage_max = 0;           // refinement: restage rows up to this long (else tile copy)
     float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
     // Light rows (fewer than dual_max ratings, and empty rows) taken by the
@@ -50,6 +51,7 @@ struct als_engine {
     als_config cfg{};
     int T = 0, ld = 0, prec = PREC_FP32, solve = SOLVE_CHOL, lds = 0;
     int dual_max = 0, dual_lds = 0, dual_grid = 0;  // dual kernel: rows below dual_max ratings (0: none)
+    int grid_v1 = 0;                                  // persistent waves of the variant-1 kernel (0: none)
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     // The dual kernel runs beside the update kernel on a stream of its own,
@@ -270,21 +272,24 @@ int launch_schedule(als_engine *e, const Side &s, const float *Y, const uint16_t
     if (s.rows == 0) return ALS_OK;
     UpdateArgs a = side_args(e, s, Y, Ys, X, Xs, lambda_rows, Lsave);
     if (solve == SOLVE_CHOL) a.refine = 0;
+    // the side's kernel variant applies to the engine's own solver
+    const int var = solve == e->solve ? s.variant : 0;
+    const int ugrid = var ? e->grid_v1 : s.grid;
     if (beside && e->dual_beside && !Lsave && s.nbatches > 0 && s.dnbatches > 0) {
         UpdateArgs d = dual_args(s, a);
         d.queue = e->queue + 2;
         HIPCHK(e, hipEventRecord(e->dual_fork, st));
         HIPCHK(e, hipStreamWaitEvent(e->dual_stream, e->dual_fork, 0));
-        HIPCHK(e, launch_update(e->T, e->prec, solve, a, s.grid, e->lds, st));
+        HIPCHK(e, launch_update(e->T, e->prec, solve, var, a, ugrid, e->lds, st));
         HIPCHK(e, launch_dual(e->T, e->prec, solve, d, e->dual_grid, e->dual_lds, e->dual_stream));
         HIPCHK(e, hipEventRecord(e->dual_join, e->dual_stream));
         HIPCHK(e, hipStreamWaitEvent(st, e->dual_join, 0));
         return ALS_OK;
     }
-    if (s.nbatches > 0) HIPCHK(e, launch_update(e->T, e->prec, solve, a, s.grid, e->lds, st));
+    if (s.nbatches > 0) HIPCHK(e, launch_update(e->T, e->prec, solve, var, a, ugrid, e->lds, st));
     if (s.dnbatches > 0) {
         const UpdateArgs d = dual_args(s, a);
-        if (Lsave) HIPCHK(e, launch_update(e->T, e->prec, solve, d, s.grid, e->lds, st));
+        if (Lsave) HIPCHK(e, launch_update(e->T, e->prec, solve, var, d, ugrid, e->lds, st));
         else HIPCHK(e, launch_dual(e->T, e->prec, solve, d, e->dual_grid, e->dual_lds, st));
     }
     return ALS_OK;
@@ -442,6 +447,12 @@ int als_create(const als_config *cfg, als_engine **out) {
         // restaging a row moves 2 FP bytes per rating; a round trip of the
         // tile copy 2 NT KiB: equal at 32 (T + 1) ratings
         e->side[0].restage_max = e->side[1].restage_max = 32 * (e->T + 1);
+    }
+    if (const void *k1 = rc ? nullptr : update_kernel_ptr(e->T, e->prec, e->solve, 1)) {
+        if (e->lds > 65536) (void)hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, e->lds);
+        int pc1 = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc1, k1, 64, e->lds) == hipSuccess && pc1 > 0)
+            e->grid_v1 = pc1 * e->num_cu;
     }
     const void *dk = dual_kernel_ptr(e->T, e->prec, e->solve);
     if (!rc && dk) {
@@ -757,6 +768,10 @@ int als_get_info(als_engine *e, als_info *o) {
     o->dual_max = e->dual_max;
     o->dual_rows[0] = e->side[0].dnunits;
     o->dual_rows[1] = e->side[1].dnunits;
+    for (int sd = 0; sd < 2; sd++) {
+        o->kernel_variant[sd] = e->side[sd].variant;
+        o->grid_variant1 = e->grid_v1;
+    }
     return ALS_OK;
 }
 
@@ -933,6 +948,19 @@ extern "C" int als_set_schedule(als_engine *e, int sd, int64_t split_len, int64_
     return ALS_OK;
 }
 
+// Selects the update kernel variant of one side (0: default, 1: one wave per
+// SIMD where it is built for the engine's solver and tile count).
+extern "C" int als_set_kernel_variant(als_engine *e, int sd, int variant) {
+    if (!e) return ALS_E_INVALID;
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (variant != 0 && variant != 1) return set_err(e, ALS_E_INVALID, "variant must be 0 or 1");
+    if (variant == 1 && e->grid_v1 == 0)
+        return set_err(e, ALS_E_UNSUPPORTED, "no one-wave kernel for this solver and tile count");
+    drop_graph(e);
+    e->side[sd].variant = variant;
+    return ALS_OK;
+}
+
 // Autotuner: times one half-sweep per candidate schedule (split length x
 // batch size, around the cost-model defaults) on the engine's own data and
 // keeps the fastest. The side's factors and the failure count are restored
@@ -952,6 +980,8 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
         if (!(sides & (1 << sd)) || s.rows == 0) {
             res.split_len[sd] = s.split_len;
             res.batch_work[sd] = s.batch_work;
+            res.restage_max[sd] = s.restage_max;
+            res.kernel_variant[sd] = s.variant;
             continue;
         }
         const int64_t rows = s.rows;
@@ -1033,9 +1063,36 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             s.restage_max = bestr;
             best = std::min(best, bestt);
         }
+        // then the kernel variant (one wave per SIMD against two) on the chosen schedule
+        if (!rc && e->grid_v1 > 0) {
+            const int v0 = s.variant;
+            int bestv = v0;
+            float bestvt = 1e30f;
+            for (int v : {v0, 1 - v0}) {
+                s.variant = v;
+                float t = 1e30f;
+                for (int rep = 0; rep < 3 && !rc; rep++) {
+                    if ((rc = restore())) break;
+                    HIPCHK(e, hipEventRecord(ev[0], e->stream));
+                    if ((rc = update_side(e, sd))) break;
+                    HIPCHK(e, hipEventRecord(ev[1], e->stream));
+                    HIPCHK(e, hipEventSynchronize(ev[1]));
+                    float ms = 0.f;
+                    HIPCHK(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
+                    if (rep > 0) t = std::min(t, ms);
+                }
+                if (t < bestvt) {
+                    bestvt = t;
+                    bestv = v;
+                }
+            }
+            s.variant = bestv;
+            best = std::min(best, bestvt);
+        }
         if (!rc) rc = restore();
         if (!rc) HIPCHK(e, hipStreamSynchronize(e->stream));
         res.restage_max[sd] = s.restage_max;
+        res.kernel_variant[sd] = s.variant;
         dev_free(e, xsave);
         dev_free(e, xssave);
         dev_free(e, fsave);
@@ -1099,8 +1156,8 @@ std::vector<char> group_signature(als_group *g, int sd) {
                                  ? side_args(e, s, e->V, e->Vs, e->U, e->Us, e->side[0].lambda_rows, nullptr)
                                  : side_args(e, s, e->U, e->Us, e->V, e->Vs, e->side[1].lambda_rows, nullptr);
         put(&a, sizeof(a));
-        const int v[] = {e->T, e->prec, e->solve, s.grid, e->lds, s.nbatches, s.dnbatches, e->dual_grid,
-                         e->dual_lds};
+        const int v[] = {e->T, e->prec, e->solve, s.grid, s.variant, e->grid_v1, e->lds, s.nbatches, s.dnbatches,
+                         e->dual_grid, e->dual_lds};
         put(v, sizeof(v));
         put(&s.dunits, sizeof(s.dunits));
         put(&s.dbatch, sizeof(s.dbatch));

@@ -5,21 +5,21 @@
 
 namespace als {
 
-const void *update_kernel_ptr_p0(int T, int solve);
-const void *update_kernel_ptr_p1(int T, int solve);
-const void *update_kernel_ptr_p2(int T, int solve);
+const void *update_kernel_ptr_p0(int T, int solve, int variant);
+const void *update_kernel_ptr_p1(int T, int solve, int variant);
+const void *update_kernel_ptr_p2(int T, int solve, int variant);
 int update_lds_bytes_p0(int T, int solve, int refine);
 int update_lds_bytes_p1(int T, int solve, int refine);
 int update_lds_bytes_p2(int T, int solve, int refine);
-hipError_t launch_update_p0(int T, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s);
-hipError_t launch_update_p1(int T, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s);
-hipError_t launch_update_p2(int T, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s);
+hipError_t launch_update_p0(int T, int solve, int variant, const UpdateArgs &a, int grid, int lds, hipStream_t s);
+hipError_t launch_update_p1(int T, int solve, int variant, const UpdateArgs &a, int grid, int lds, hipStream_t s);
+hipError_t launch_update_p2(int T, int solve, int variant, const UpdateArgs &a, int grid, int lds, hipStream_t s);
 
-const void *update_kernel_ptr(int T, int prec, int solve) {
+const void *update_kernel_ptr(int T, int prec, int solve, int variant) {
     switch (prec) {
-    case PREC_FP32: return update_kernel_ptr_p0(T, solve);
-    case PREC_BF16: return update_kernel_ptr_p1(T, solve);
-    case PREC_FP16: return update_kernel_ptr_p2(T, solve);
+    case PREC_FP32: return update_kernel_ptr_p0(T, solve, variant);
+    case PREC_BF16: return update_kernel_ptr_p1(T, solve, variant);
+    case PREC_FP16: return update_kernel_ptr_p2(T, solve, variant);
     default: return nullptr;
     }
 }
@@ -33,11 +33,12 @@ int update_lds_bytes(int T, int prec, int solve, int refine) {
     }
 }
 
-hipError_t launch_update(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds, hipStream_t s) {
+hipError_t launch_update(int T, int prec, int solve, int variant, const UpdateArgs &a, int grid, int lds,
+                         hipStream_t s) {
     switch (prec) {
-    case PREC_FP32: return launch_update_p0(T, solve, a, grid, lds, s);
-    case PREC_BF16: return launch_update_p1(T, solve, a, grid, lds, s);
-    case PREC_FP16: return launch_update_p2(T, solve, a, grid, lds, s);
+    case PREC_FP32: return launch_update_p0(T, solve, variant, a, grid, lds, s);
+    case PREC_BF16: return launch_update_p1(T, solve, variant, a, grid, lds, s);
+    case PREC_FP16: return launch_update_p2(T, solve, variant, a, grid, lds, s);
     default: return hipErrorInvalidValue;
     }
 }

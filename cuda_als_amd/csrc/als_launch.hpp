@@ -95,9 +95,12 @@ struct RmseArgs {
 
 // Launchers (als_kernels_*.hip). Each returns the hipError_t of the launch.
 int update_lds_bytes(int T, int prec, int solve, int refine);
-hipError_t launch_update(int T, int prec, int solve, const UpdateArgs &a, int grid, int lds_bytes,
+// variant 0: the default kernel of the solver (two waves per SIMD for the
+// refinement solvers); variant 1: one wave per SIMD (warm-started refinement,
+// T >= 7; null elsewhere).
+hipError_t launch_update(int T, int prec, int solve, int variant, const UpdateArgs &a, int grid, int lds_bytes,
                          hipStream_t s);
-const void *update_kernel_ptr(int T, int prec, int solve);
+const void *update_kernel_ptr(int T, int prec, int solve, int variant = 0);
 // Light rows through the dual system, one launch per half-sweep after the
 // update kernel (null / 0 where the variant has no dual system: f < 64, CG,
 // classic refinement). dual_max_ratings: rows below it take the dual system.

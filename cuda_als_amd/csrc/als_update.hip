@@ -25,6 +25,18 @@ static const void *kernel_w2_for(int T) {
     }
 }
 
+// One wave per SIMD (no waves-per-EU bound: the whole register file for one
+// wave, spill-free at the widest tile counts); selectable per side.
+template <int S>
+static const void *kernel_w1_for(int T) {
+    switch (T) {
+    case 7: return (const void *)&update_kernel<7, ALS_PREC, S>;
+    case 8: return (const void *)&update_kernel<8, ALS_PREC, S>;
+    case 9: return (const void *)&update_kernel<9, ALS_PREC, S>;
+    default: return nullptr;
+    }
+}
+
 template <int S>
 static const void *kernel_for(int T) {
     switch (T) {
@@ -50,7 +62,9 @@ static int lds_for(int solve, int refine) {
 #define ALS_CAT2(a, b) a##b
 #define ALS_CAT(a, b) ALS_CAT2(a, b)
 
-const void *ALS_CAT(update_kernel_ptr_p, ALS_PREC)(int T, int solve) {
+const void *ALS_CAT(update_kernel_ptr_p, ALS_PREC)(int T, int solve, int variant) {
+    if (variant == 1) return solve == SOLVE_CHOL_WARM ? kernel_w1_for<SOLVE_CHOL_WARM>(T) : nullptr;
+    if (variant != 0) return nullptr;
     return solve == SOLVE_CHOL        ? kernel_for<SOLVE_CHOL>(T)
            : solve == SOLVE_CG        ? kernel_for<SOLVE_CG>(T)
            : solve == SOLVE_CHOL_WARM ? kernel_w2_for<SOLVE_CHOL_WARM>(T)
@@ -126,9 +140,9 @@ int ALS_CAT(dual_max_ratings_p, ALS_PREC)(int T) {
     }
 }
 
-hipError_t ALS_CAT(launch_update_p, ALS_PREC)(int T, int solve, const UpdateArgs &a, int grid, int lds,
+hipError_t ALS_CAT(launch_update_p, ALS_PREC)(int T, int solve, int variant, const UpdateArgs &a, int grid, int lds,
                                              hipStream_t s) {
-    const void *k = ALS_CAT(update_kernel_ptr_p, ALS_PREC)(T, solve);
+    const void *k = ALS_CAT(update_kernel_ptr_p, ALS_PREC)(T, solve, variant);
     if (!k) return hipErrorInvalidValue;
     void *args[] = {const_cast<UpdateArgs *>(&a)};
     return hipLaunchKernel(k, dim3(grid), dim3(WAVE), args, (size_t)lds, s);

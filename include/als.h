@@ -100,6 +100,8 @@ typedef struct {
     int64_t m, n, nnz, n_test;
     int dual_max;        /* rows with fewer ratings take the dual system (0: none) */
     int dual_rows[2];    /* rows of each side the dual system takes */
+    int kernel_variant[2]; /* update kernel variant per side (als_set_kernel_variant) */
+    int grid_variant1;   /* persistent waves of the one-wave-per-SIMD variant (0: not built) */
 } als_info;
 
 int als_config_init(als_config *cfg);
@@ -200,12 +202,19 @@ typedef struct {
     int restage_max[2];                  /* refinement: rows up to this long restage their
                                             ratings for the residual, longer ones keep a
                                             copy of the tiles (tuned with refine_steps > 0) */
+    int kernel_variant[2];               /* update kernel variant chosen (0: default, 1: one
+                                            wave per SIMD; tried where it is built) */
 } als_tune_result;
 int als_autotune(als_engine *e, int sides, als_tune_result *out);
 /* Applies one side's schedule directly (e.g. a stored als_autotune result):
  * split length, batch size (<= 0: keep) and the refinement restage threshold
  * (< 0: keep). No reference counterpart. */
 int als_set_schedule(als_engine *e, int side, int64_t split_len, int64_t batch_work, int restage_max);
+/* Update kernel variant of one side: 0 the default (two waves per SIMD for the
+ * refinement solvers), 1 one wave per SIMD with the whole register file
+ * (warm-started refinement at 7 to 9 tiles; ALS_E_UNSUPPORTED elsewhere). The
+ * results are identical; als_autotune times both. No reference counterpart. */
+int als_set_kernel_variant(als_engine *e, int side, int variant);
 
 /* Model groups (no reference counterpart; the reference trains one model per
  * run): up to 4 engines over the same rating matrix (same m, n, nnz and
