@@ -615,13 +615,31 @@ __device__ __forceinline__ void diag_factor(f32x4 &d, int jend) {
     for (int r = 0; r < 4; ++r) d[r] = (FULL || c < jend) ? w[r] * myrs : 0.f;
 }
 
+// A tile's four registers as FP16 hi and lo parts, x = hi + lo + O(2^-22 x):
+// the operand of a 16x16x16 FP16 MFMA in the register layout of the FP32
+// 16x16x4 form (lane (q, c) holds k = 4q .. 4q+3 of row c).
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void split16(const f32x4 &x, f16x4 &hi, f16x4 &lo) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const _Float16 h = (_Float16)x[r];
+        hi[r] = h;
+        lo[r] = (_Float16)(x[r] - (float)h);
+    }
+}
+
 // Blocked right-looking Cholesky, panel K:
 //   diagonal block -> W_KK = L_KK^-1 (diag_factor)
 //   panel          L_IK^T = W_KK A_IK^T       (4 MFMAs per tile)
-//   trailing       A_IJ  -= L_IK L_JK^T        (4 MFMAs per tile)
+//   trailing       A_IJ  -= L_IK L_JK^T        (4 FP32 MFMAs per tile; with
+//                  SPLIT three FP16 16x16x16 MFMAs on the hi/lo parts of
+//                  the panel tiles, hi hi + hi lo + lo hi, each FP32 product
+//                  kept to ~2^-20 relative at an eighth of the MFMA time;
+//                  the entries of L are bounded by sqrt(A_ii), well inside
+//                  FP16's range)
 // The last block holds row f (the augmented row, never a pivot): yl gets
 // its reduced entries (row form) just before that block is factored.
-template <int T, int K, int NT>
+template <int T, int K, int NT, bool SPLIT>
 __device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, f32x4 &yl) {
     if constexpr (K < T) {
         const int jend = min(16, f - 16 * K);
@@ -641,28 +659,47 @@ __device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, f32x4 &yl) {
                     nw = __builtin_amdgcn_mfma_f32_16x16x4f32(a[tile_id(K, K)][r], o[r], nw, 0, 0, 0);
                 a[tile_id(I, K)] = nw;
             }
+            if constexpr (SPLIT) {
+                f16x4 hi[T], lo[T];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+                for (int I = K + 1; I < T; ++I) split16(a[tile_id(I, K)], hi[I], lo[I]);
 #pragma unroll
                 for (int J = K + 1; J < T; ++J) {
-                    const float nf = -a[tile_id(J, K)][r];
+                    const f16x4 nh = -hi[J], nl = -lo[J];
 #pragma unroll
-                    for (int I = J; I < T; ++I)
-                        a[tile_id(I, J)] =
-                            __builtin_amdgcn_mfma_f32_16x16x4f32(nf, a[tile_id(I, K)][r], a[tile_id(I, J)], 0, 0, 0);
+                    for (int I = J; I < T; ++I) {
+                        f32x4 t = a[tile_id(I, J)];
+                        t = __builtin_amdgcn_mfma_f32_16x16x16f16(nh, hi[I], t, 0, 0, 0);
+                        t = __builtin_amdgcn_mfma_f32_16x16x16f16(nh, lo[I], t, 0, 0, 0);
+                        t = __builtin_amdgcn_mfma_f32_16x16x16f16(nl, hi[I], t, 0, 0, 0);
+                        a[tile_id(I, J)] = t;
+                    }
                 }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int J = K + 1; J < T; ++J) {
+                        const float nf = -a[tile_id(J, K)][r];
+#pragma unroll
+                        for (int I = J; I < T; ++I)
+                            a[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x4f32(nf, a[tile_id(I, K)][r],
+                                                                                   a[tile_id(I, J)], 0, 0, 0);
+                    }
+            }
         }
-        chol_panels<T, K + 1>(a, f, yl);
+        chol_panels<T, K + 1, NT, SPLIT>(a, f, yl);
     }
 }
 
 // Factorises the augmented matrix in place and returns z = L^-1 b in row
 // form (zero past f). A pivot <= 0 leaves non-finite values that reach x
 // (the caller checks x); the return value is kept for that interface.
-template <int T, int NT>
+// SPLIT: the trailing updates on FP16 hi/lo parts (chol_panels).
+template <int T, int NT, bool SPLIT = false>
 __device__ __forceinline__ bool chol_regs(f32x4 (&a)[NT], int f, f32x4 (&zr)[T]) {
     f32x4 yl;
-    chol_panels<T, 0>(a, f, yl);
+    chol_panels<T, 0, NT, SPLIT>(a, f, yl);
     const int l = lane(), q = l >> 4, c = l & 15;
 #pragma unroll
     for (int K = 0; K + 1 < T; ++K)
@@ -1246,7 +1283,7 @@ __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n,
         for (int t = 0; t < NTD; ++t) *reinterpret_cast<f32x4 *>(kst + t * 256 + 4 * l) = acc[t];
     }
     f32x4 zr[TD];
-    chol_regs<TD>(acc, n, zr);
+    chol_regs<TD, NTD, (E != PREC_FP32)>(acc, n, zr);
     float al[TD];
     back_solve<TD>(acc, n, zr, al);
     if constexpr (REFINE) {
@@ -1402,7 +1439,8 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                         }
                     }
                     f32x4 zr[T];
-                    ok = chol_regs<T>(acc, f, zr);
+                    // the 16-bit-Gram paths factor with split-FP16 trailing updates
+                    ok = chol_regs<T, G::NT, (E != PREC_FP32)>(acc, f, zr);
                     back_solve<T>(acc, f, zr, x);
                     if (warm) {
                         // x0 again from X (not kept live across the factorisation)
