@@ -132,6 +132,29 @@ int check_csr(als_engine *e, int64_t rows, int64_t cols, int64_t nnz, const int 
     return ALS_OK;
 }
 
+// The 16-bit-Gram paths take the Cholesky's trailing updates as FP16
+// products of split operands (high part and remainder), so every entry of
+// the factor must stay inside FP16's range (65504). The rating row of the
+// augmented factor holds z = L^-1 b with |z|^2 = b^T A^-1 b <= sum r^2
+// (primal system) and <= sum r^2 / (lambda n) (dual system of a light row,
+// whose matrix is at least lambda n I). Rows whose bound reaches 2^31
+// (|z| >= 46341) are refused; the FP32 path has no such limit. The Gram
+// rows themselves are bounded by sqrt(A_ii), A_ii = sum y_i^2 + lambda n.
+int check_fp16_range(als_engine *e, int64_t rows, const int *ptr, const float *val, const char *what) {
+    const double lim = 2147483648.0;
+    for (int64_t r = 0; r < rows; r++) {
+        double s2 = 0;
+        for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) s2 += (double)val[k] * val[k];
+        const double ln = (double)e->cfg.lambda * (ptr[r + 1] - ptr[r]);
+        if (!(s2 < lim) || (ln > 0 && ln < 1 && !(s2 / ln < lim)))
+            return set_err(e, ALS_E_INVALID,
+                           std::string(what) + " row " + std::to_string(r) +
+                               ": its ratings exceed the 16-bit Gram's range (sum of squares, over lambda n "
+                               "where that is below 1, must stay below 2^31); use the FP32 Gram");
+    }
+    return ALS_OK;
+}
+
 // Frees one side's schedule buffers.
 void free_schedule(als_engine *e, Side &s) {
     dev_free(e, s.units);
@@ -525,6 +548,8 @@ int als_set_train(als_engine *e, int64_t m, int64_t n, int64_t nnz, const int *c
         return set_err(e, ALS_E_INVALID, "null train array");
     int rc = check_csr(e, m, n, nnz, csr_ptr, csr_idx, "CSR");
     if (!rc) rc = check_csr(e, n, m, nnz, csc_ptr, csc_idx, "CSC");
+    if (!rc && e->prec != PREC_FP32) rc = check_fp16_range(e, m, csr_ptr, csr_val, "CSR");
+    if (!rc && e->prec != PREC_FP32) rc = check_fp16_range(e, n, csc_ptr, csc_val, "CSC");
     if (rc) return rc;
     if (e->have_train) return set_err(e, ALS_E_STATE, "train data already set");
     drop_graph(e);

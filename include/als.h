@@ -122,7 +122,10 @@ const char *als_last_error(const als_engine *e);
  * is ordered before and after it as usual. */
 int als_set_stream(als_engine *e, void *hip_stream);
 
-/* Host arrays in the reference layout (cuda_sparse_matrix.h:29-37); copied. */
+/* Host arrays in the reference layout (cuda_sparse_matrix.h:29-37); copied.
+ * With a 16-bit Gram (ALS_PREC_FP16 / ALS_PREC_BF16) every row's sum of
+ * squared ratings, divided by lambda * n where that product is below 1, must
+ * stay below 2^31 (the factorisation's FP16 range); ALS_E_INVALID otherwise. */
 int als_set_train(als_engine *e, int64_t m, int64_t n, int64_t nnz,
                   const int *csr_ptr, const int *csr_idx, const float *csr_val,
                   const int *csc_ptr, const int *csc_idx, const float *csc_val);

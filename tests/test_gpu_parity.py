@@ -300,6 +300,35 @@ def test_empty_rows_get_zero_factors():
         assert np.max(np.abs(Xg - Xn)) <= 1e-4 * max(1.0, np.abs(Xn).max())
 
 
+def test_16bit_gram_refuses_ratings_beyond_fp16_range():
+    """The 16-bit-Gram paths factor with FP16 products, so a row whose
+    ratings would put |L^-1 b| (at most sqrt(sum r^2), or sqrt(sum r^2 /
+    (lambda n)) below lambda n = 1) outside FP16's range is refused at
+    als_set_train; the FP32 path takes the same data."""
+    rng = np.random.default_rng(3)
+    m, n = 40, 30
+    dense = (rng.random((m, n)) < 0.3) * rng.integers(1, 6, (m, n)).astype(np.float32)
+    dense[0, :10] = 20000.0  # sum of squares 4e9 on row 0
+    rows, cols = np.nonzero(dense)
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m))])
+    d = ca.from_arrays(m, n, ptr, cols, dense[rows, cols], rows[:3], cols[:3], dense[rows[:3], cols[:3]])
+    for prec in (E.Prec.FP16, E.Prec.BF16):
+        with pytest.raises(ca.AlsError, match="16-bit Gram"):
+            E.Engine(E.make_config(16, 0.05, 1, E.SolveType.CHOL, prec), d)
+    with E.Engine(E.make_config(16, 0.05, 1, E.SolveType.CHOL, E.Prec.FP32), d) as e:
+        e.init_factors()
+        e.sweep()
+        assert np.isfinite(e.get_factors(0)).all()
+    # 1000-star ratings stay inside the range
+    dense[0, :10] = 1000.0
+    rows, cols = np.nonzero(dense)
+    d = ca.from_arrays(m, n, ptr, cols, dense[rows, cols], rows[:3], cols[:3], dense[rows[:3], cols[:3]])
+    with E.Engine(E.make_config(16, 0.05, 1, E.SolveType.CHOL, E.Prec.FP16), d) as e:
+        e.init_factors()
+        e.sweep()
+        assert e.info().failures == 0 and np.isfinite(e.get_factors(0)).all()
+
+
 @pytest.mark.parametrize("f,warm", [(16, False), (100, True)])
 def test_graph_replay_matches_direct(toy, f, warm):
     """Each sweep replayed from the captured graph (with the RMSE and host
