@@ -354,7 +354,8 @@ def main() -> int:
                    "solve": (f"fp32 Cholesky + {a.refine} fp64-residual refinement step(s)"
                              + (", the first from the previous iterate" if a.refine_warm and a.refine else "")
                              + (f"; rows under {info_m.dual_max} ratings ({info_m.dual_rows[0]} users, "
-                                f"{info_m.dual_rows[1]} items) through their n x n dual system"
+                                f"{info_m.dual_rows[1]} items) through their n x n dual system (fp32 "
+                                f"Cholesky, no refinement step)"
                                 if info_m.dual_max else "")),
                    "timing": "whole sweeps replayed from the captured sweep graph; per-half times from a "
                              "second pass launched half by half",

@@ -115,12 +115,10 @@ template <int T, int E> struct Geo {
     // instruction share four).
     __host__ __device__ static constexpr int yt_row(int j) { return j + (j >> 3); }
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
-    // Rows with fewer than 16 * DUAL_TD ratings take the dual system
-    // (T >= 5), whose FP32 tile copy lives in the stage.
+    // Rows with fewer than 16 * DUAL_TD ratings take the dual system (T >= 5).
     static constexpr int DUAL_TD = T >= 5 ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
     static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : yt_row(FP) * YT_H * 2;
-    static constexpr int DUAL_STAGE_BYTES = DUAL_TD * (DUAL_TD + 1) / 2 * 1024;
-    // LDS of the update kernels (Gram stage) and of the dual kernel (tile copy)
+    // LDS of the update kernels (misc + Gram stage) and of the dual kernel (misc)
     // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
     static constexpr int MISC_FLOATS = 32 + 32 + FP + 64;
     __host__ __device__ static constexpr int lds_bytes() { return MISC_FLOATS * 4 + GRAM_STAGE_BYTES; }
@@ -129,7 +127,7 @@ template <int T, int E> struct Geo {
     __host__ __device__ static constexpr int refine_lds_bytes() {
         return MISC_FLOATS * 4 + (GRAM_STAGE_BYTES > FP32_STAGE_BYTES ? GRAM_STAGE_BYTES : FP32_STAGE_BYTES);
     }
-    __host__ __device__ static constexpr int dual_lds_bytes() { return MISC_FLOATS * 4 + DUAL_STAGE_BYTES; }
+    __host__ __device__ static constexpr int dual_lds_bytes() { return MISC_FLOATS * 4; }
 };
 
 // ------------------------------------------------------------------ Gram
@@ -1205,15 +1203,16 @@ __device__ __forceinline__ void dual_operands(const UpdateArgs &a, const int (&i
 }
 
 // One row through the dual system: x (column form of the T-tile layout,
-// zero past f). With REFINE, one refinement step: the FP64 residual
-// r - K alpha from an FP32 copy of the tiles taken before the
-// factorisation (kst: NTD KB of LDS), the correction solved on the factor.
+// zero past f), a plain FP32 Cholesky solve of the 16-bit-formed (or FP32)
+// dual matrix. Its matrix is at least lambda n I and its condition number at
+// most 1 + |y|^2 / lambda over the rated rows, so the FP32 solve's error
+// stays far below the rounding of the 16-bit Gram that every row of the
+// mixed path carries; no refinement step is taken here.
 // xs: 16T floats of this wave's LDS.
-template <int TD, int T, int E, bool REFINE>
+template <int TD, int T, int E>
 __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n, float reg, float (&xc)[T],
-                                         float *xs, float *kst) {
+                                         float *xs) {
     constexpr int NTD = TD * (TD + 1) / 2;
-    static_assert(!REFINE || NTD * 1024 <= Geo<T, E>::DUAL_STAGE_BYTES, "tile copy must fit the stage");
     int l = lane();
     // Opaque per row: lane masks of this path are formed here, not hoisted
     // out of the row loop where they would stay live across the other path.
@@ -1277,48 +1276,10 @@ __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n,
             acc[D][r] = v;
         }
     }
-    if constexpr (REFINE) {
-        // FP32 copy of the tiles for the residual, in the stage (unused here)
-#pragma unroll
-        for (int t = 0; t < NTD; ++t) *reinterpret_cast<f32x4 *>(kst + t * 256 + 4 * l) = acc[t];
-    }
     f32x4 zr[TD];
     chol_regs<TD, NTD, (E != PREC_FP32)>(acc, n, zr);
     float al[TD];
     back_solve<TD>(acc, n, zr, al);
-    if constexpr (REFINE) {
-        // e = r - K alpha in FP64 (alpha is zero from index n on, so row and
-        // column n of the copy add nothing to the rows kept)
-        if (l < 16) {
-#pragma unroll
-            for (int J = 0; J < TD; ++J) xs[16 * J + l] = al[J];
-        }
-        wave_sync();
-        float ec[TD];
-#pragma unroll
-        for (int K = 0; K < TD; ++K) {
-            double cp = 0.0;
-#pragma unroll
-            for (int J = 0; J <= K; ++J) {
-                const f32x4 aJ = *reinterpret_cast<const f32x4 *>(xs + 16 * J + 4 * q);
-                const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(K, J) * 256 + 4 * l);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) cp += (double)kt[r] * (double)aJ[r];
-            }
-            double rp[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int I = K + 1; I < TD; ++I) {
-                const f32x4 kt = *reinterpret_cast<const f32x4 *>(kst + tile_id(I, K) * 256 + 4 * l);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) rp[r] += (double)kt[r] * (double)al[I];
-            }
-            const double rsum = (K + 1 < TD) ? rows16_scatter_d(rp) : 0.0;
-            const double y = q4_sum_d(cp) + __shfl(rsum, ((c >> 2) << 4) | c, WAVE);
-            ec[K] = (16 * K + c < n) ? (float)((double)rc[K] - y) : 0.f;
-        }
-        correct<TD>(acc, n, ec, al);
-        wave_sync();  // every lane has read alpha from xs before x is written there
-    }
     // x = Y^T alpha: lane (q, c) multiplies segment 4s + q of row 16I + c by
     // alpha[16I + c]; after the scattered sum over c, lane c holds factor
     // 8 (4s + q) + c / 2 (16-bit) or 4 (4s + q) + c % 4 (FP32).
@@ -1354,14 +1315,14 @@ __device__ __forceinline__ void dual_row(const UpdateArgs &a, int64_t rs, int n,
 }
 
 // The dual system of the fewest tiles that hold n + 1 rows.
-template <int TD, int TDMAX, int T, int E, bool REFINE>
+template <int TD, int TDMAX, int T, int E>
 __device__ __forceinline__ void dual_pick(const UpdateArgs &a, int64_t rs, int n, float reg, float (&xc)[T],
-                                          float *xs, float *kst) {
+                                          float *xs) {
     if constexpr (TD < TDMAX) {
-        if (n < 16 * TD) dual_row<TD, T, E, REFINE>(a, rs, n, reg, xc, xs, kst);
-        else dual_pick<TD + 1, TDMAX, T, E, REFINE>(a, rs, n, reg, xc, xs, kst);
+        if (n < 16 * TD) dual_row<TD, T, E>(a, rs, n, reg, xc, xs);
+        else dual_pick<TD + 1, TDMAX, T, E>(a, rs, n, reg, xc, xs);
     } else {
-        dual_row<TD, T, E, REFINE>(a, rs, n, reg, xc, xs, kst);
+        dual_row<TD, T, E>(a, rs, n, reg, xc, xs);
     }
 }
 
@@ -1521,7 +1482,6 @@ __global__ __launch_bounds__(WAVE) void dual_kernel(UpdateArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int l = lane();
     float *xs = reinterpret_cast<float *>(lds) + 64;
-    float *kst = reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4);
     for (;;) {
         const int b = pop_batch(a.queue);
         if (b >= a.nbatches) break;
@@ -1534,7 +1494,7 @@ __global__ __launch_bounds__(WAVE) void dual_kernel(UpdateArgs a) {
             for (int J = 0; J < T; ++J) x[J] = 0.f;
             if (n > 0) {
                 const float reg = (a.lambda_rows ? a.lambda_rows[row] : a.lambda) * (float)n;
-                dual_pick<1, G::DUAL_TD, T, E, S == SOLVE_CHOL_WARM>(a, a.ptr[row], n, reg, x, xs, kst);
+                dual_pick<1, G::DUAL_TD, T, E>(a, a.ptr[row], n, reg, x, xs);
                 bool finite = true;
 #pragma unroll
                 for (int J = 0; J < T; ++J) finite = finite && isfinite(x[J]);

@@ -10,9 +10,9 @@ one FP64-residual refinement step, classic or warm-started) 5e-5 against the
 FP64 solve of the system the refinement converges to: rows whose residual is
 restaged from the FP32 master rows (classic, whole rows up to 256 ratings)
 against the exact system; rows refined against their FP32-accumulated 16-bit
-tiles (long and split rows, every warm-started row, and the light rows solved
-through their dual system) against the FP16-rounded system, whose
-accumulation error they keep (~1e-5 on the heaviest items).
+tiles (long and split rows, every warm-started row) and the light rows solved
+through their dual system (a plain FP32 solve) against the FP16-rounded
+system, whose accumulation error they keep (~1e-5 on the heaviest items).
 FP32 path 1e-4 against the exact FP64 solve.
 """
 from pathlib import Path

@@ -567,12 +567,14 @@ def test_warm_started_refinement(toy):
 def test_dual_system_for_light_rows(toy, f, prec, oprec):
     """With a 16-bit Gram and one warm-started refinement step, whole rows
     with fewer than 16 min(T - 2, 5) ratings (48 at f = 64, 80 at f = 100)
-    are solved in the space of their ratings, x = Y^T (Y Y^T + mu I)^-1 r.
-    Every user of the toy workload (at most 40 ratings) and most of its
-    items take that path. One
-    half-sweep of each side from the oracle's trained state matches the
-    FP64 solve of the 16-bit-rounded system within 1e-5 relative to
-    max(1, |x|); the rows through the dual system land within ~2e-7."""
+    are solved in the space of their ratings, x = Y^T (Y Y^T + mu I)^-1 r,
+    by a plain FP32 Cholesky solve (the refinement step is the primal
+    rows'). Every user of the toy workload (at most 40 ratings) and most of
+    its items take that path. One half-sweep of each side from the oracle's
+    trained state matches the FP64 solve of the 16-bit-rounded system within
+    1e-5 relative to max(1, |x|); the rows through the dual system land
+    within ~3e-7 (bar 1e-6): its matrix is at least lambda n I, so the FP32
+    solve needs no refinement there."""
     assert np.diff(toy.csr_ptr).max() < 48
     U0, V0 = O.reference_init(toy.m, toy.n, f)
     Uo, Vo, _, _ = O.train(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=oprec, store_fp32=True), toy, 3, U0, V0)
@@ -604,8 +606,9 @@ def test_dual_system_tile_boundaries(f, prec, refine, warm, oprec, tol):
     31, 32, 47, 48, 63, 64, 79), the one-rating row, the empty row, and rows
     just past the dual limit (48 at f = 64, 80 at f = 100) or far past it,
     which take the primal system. One users half-sweep from random factors
-    against the FP64 oracle (relative to max(1, |x|)); plain solves 1e-4,
-    warm-started 1e-5."""
+    against the FP64 oracle (relative to max(1, |x|)); plain solves 1e-4;
+    with the warm-started step 1e-5 (its primal rows refined, its dual rows
+    a plain FP32 solve of a system at least lambda n I)."""
     rng = np.random.default_rng(17)
     counts = np.repeat([0, 1, 2, 14, 15, 16, 17, 30, 31, 32, 33, 46, 47, 48, 49, 63, 64, 65, 79, 80, 81, 100], 3)
     m, n_items = len(counts), 400
