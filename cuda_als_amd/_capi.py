@@ -56,7 +56,7 @@ class ScheduleStats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("split_len", C.c_int64 * 2), ("batch_work", C.c_int64 * 2), ("default_ms", C.c_float * 2),
                 ("best_ms", C.c_float * 2), ("candidates", C.c_int * 2), ("restage_max", C.c_int * 2),
-                ("kernel_variant", C.c_int * 2)]
+                ("kernel_variant", C.c_int * 2), ("grid", C.c_int * 2)]
 
 
 class SynthSpec(C.Structure):
@@ -106,6 +106,7 @@ SIGNATURES = {
     "als_autotune": (C.c_int, [P, C.c_int, C.POINTER(TuneResult)]),
     "als_set_schedule": (C.c_int, [P, C.c_int, C.c_int64, C.c_int64, C.c_int]),
     "als_set_kernel_variant": (C.c_int, [P, C.c_int, C.c_int]),
+    "als_set_grid": (C.c_int, [P, C.c_int, C.c_int]),
     "als_group_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
     "als_group_destroy": (None, [P]),
     "als_group_last_error": (C.c_char_p, [P]),

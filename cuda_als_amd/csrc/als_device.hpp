@@ -1475,9 +1475,11 @@ __global__ __launch_bounds__(WAVE) void update_kernel(UpdateArgs a) {
 // 16 * DUAL_TD ratings (and empty rows, x = 0), the units of their own
 // schedule, launched beside the update kernel of the same half-sweep. It
 // carries none of the primal Gram and solve code, and the update kernel
-// none of its code.
+// none of its code. Four waves per SIMD (128 VGPRs; at T = 8 and 9 one or
+// two registers spill): its row is mostly VALU work on few live registers,
+// and four waves run it faster than three (users half -1.8% at f = 100).
 template <int T, int E, int S>
-__global__ __launch_bounds__(WAVE) void dual_kernel(UpdateArgs a) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4, 4))) void dual_kernel(UpdateArgs a) {
     using G = Geo<T, E>;
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int l = lane();

@@ -34,7 +34,8 @@ struct Side {
     float *partial = nullptr;
     int nbatches = 0, nunits = 0, nsplits = 0;
     int64_t nslots = 0, split_len = 0, batch_work = 0;
-    int grid = 0;                  // persistent waves of the default update kernel (variant 0)
+    int grid = 0;                  // persistent waves of the default update kernel (variant 0),
+                                   // at most the engine's grid_max (autotuned per side)
     int variant = 0;               // update kernel variant (1: one wave per SIMD, where built)
     int restage_max = 0;           // refinement: restage rows up to this long (else tile copy)
     float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
@@ -52,6 +53,7 @@ struct als_engine {
     int T = 0, ld = 0, prec = PREC_FP32, solve = SOLVE_CHOL, lds = 0;
     int dual_max = 0, dual_lds = 0, dual_grid = 0;  // dual kernel: rows below dual_max ratings (0: none)
     int grid_v1 = 0;                                  // persistent waves of the variant-1 kernel (0: none)
+    int grid_max = 0;                                 // occupancy limit of the default update kernel
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     // The dual kernel runs beside the update kernel on a stream of its own,
@@ -466,7 +468,7 @@ int als_create(const als_config *cfg, als_engine **out) {
         rc = set_err(e, ALS_E_HIP, "occupancy query failed");
     if (!rc) {
         per_cu = std::max(1, per_cu);
-        e->side[0].grid = e->side[1].grid = per_cu * e->num_cu;
+        e->side[0].grid = e->side[1].grid = e->grid_max = per_cu * e->num_cu;
         // restaging a row moves 2 FP bytes per rating; a round trip of the
         // tile copy 2 NT KiB: equal at 32 (T + 1) ratings
         e->side[0].restage_max = e->side[1].restage_max = 32 * (e->T + 1);
@@ -487,7 +489,7 @@ int als_create(const als_config *cfg, als_engine **out) {
         e->dual_max = dual_max_ratings(e->T, e->prec);
     }
     if (!rc && e->solve == SOLVE_CHOL_REFINE)
-        rc = dev_alloc(e, &e->scratch, (size_t)e->side[0].grid * (e->T * (e->T + 1) / 2) * 256);
+        rc = dev_alloc(e, &e->scratch, (size_t)e->grid_max * (e->T * (e->T + 1) / 2) * 256);
     // Per-dispatch hardware counters (rocprofv3 --pmc) need the two kernels
     // of a half-sweep one after the other: ALS_DUAL_SEQUENTIAL=1 keeps them so.
     if (const char *v = std::getenv("ALS_DUAL_SEQUENTIAL")) e->dual_beside = std::atoi(v) == 0;
@@ -986,9 +988,26 @@ extern "C" int als_set_kernel_variant(als_engine *e, int sd, int variant) {
     return ALS_OK;
 }
 
+// Persistent waves of the default update kernel on one side (1 ..
+// grid_max); the schedule is rebuilt for that many waves.
+extern "C" int als_set_grid(als_engine *e, int sd, int grid) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (grid < 1 || grid > e->grid_max)
+        return set_err(e, ALS_E_INVALID, "grid must be between 1 and " + std::to_string(e->grid_max));
+    drop_graph(e);
+    Side &s = e->side[sd];
+    s.grid = grid;
+    return build_side(e, s, s.rows, e->hptr[sd].data(), s.split_len, s.batch_work);
+}
+
 // Autotuner: times one half-sweep per candidate schedule (split length x
 // batch size, around the cost-model defaults) on the engine's own data and
-// keeps the fastest. The side's factors and the failure count are restored
+// keeps the fastest; then the update kernel's grid (fewer persistent waves
+// than the occupancy limit can gather faster: the items half of the
+// netflix-shaped workload runs 4% faster at six waves per CU than at
+// eight), the restage threshold and the kernel variant. The side's factors and the failure count are restored
 // after every trial, so tuning leaves the model state unchanged.
 extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
     if (!e) return ALS_E_INVALID;
@@ -1007,6 +1026,7 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             res.batch_work[sd] = s.batch_work;
             res.restage_max[sd] = s.restage_max;
             res.kernel_variant[sd] = s.variant;
+            res.grid[sd] = s.grid;
             continue;
         }
         const int64_t rows = s.rows;
@@ -1040,21 +1060,28 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             HIPCHK(e, hipMemcpyAsync(e->fail, fsave, sizeof(int), hipMemcpyDeviceToDevice, e->stream));
             return ALS_OK;
         };
-        float best = 1e30f;
-        size_t besti = 0;
-        for (size_t i = 0; i < cand.size() && !rc; i++) {
-            if ((rc = build_side(e, s, rows, hp, cand[i].first, cand[i].second))) break;
-            float t = 1e30f;
-            for (int rep = 0; rep < 3 && !rc; rep++) {  // first run warms caches, two timed
-                if ((rc = restore())) break;
+        // one candidate: a run that warms the caches, then the faster of two timed runs
+        auto trial = [&](float &t) -> int {
+            t = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                int r;
+                if ((r = restore())) return r;
                 HIPCHK(e, hipEventRecord(ev[0], e->stream));
-                if ((rc = update_side(e, sd))) break;
+                if ((r = update_side(e, sd))) return r;
                 HIPCHK(e, hipEventRecord(ev[1], e->stream));
                 HIPCHK(e, hipEventSynchronize(ev[1]));
                 float ms = 0.f;
                 HIPCHK(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
                 if (rep > 0) t = std::min(t, ms);
             }
+            return ALS_OK;
+        };
+        float best = 1e30f;
+        size_t besti = 0;
+        for (size_t i = 0; i < cand.size() && !rc; i++) {
+            if ((rc = build_side(e, s, rows, hp, cand[i].first, cand[i].second))) break;
+            float t;
+            if ((rc = trial(t))) break;
             if (i == 0) res.default_ms[sd] = t;
             if (t < best) {
                 best = t;
@@ -1062,6 +1089,25 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             }
         }
         if (!rc) rc = build_side(e, s, rows, hp, cand[besti].first, cand[besti].second);
+        // then the grid: the occupancy limit and one or two waves per CU fewer
+        // (down to three quarters of it), the schedule rebuilt for each
+        if (!rc) {
+            const int pc = e->grid_max / e->num_cu, g0 = s.grid;
+            int bestg = g0;
+            for (int k = 1; k <= 2 && !rc; k++) {
+                const int g = (pc - k) * e->num_cu;
+                if (4 * (pc - k) < 3 * pc || pc - k < 1) break;
+                s.grid = g;
+                float t;
+                if ((rc = build_side(e, s, rows, hp, s.split_len, s.batch_work)) || (rc = trial(t))) break;
+                if (t < best) {
+                    best = t;
+                    bestg = g;
+                }
+            }
+            s.grid = bestg;
+            if (!rc) rc = build_side(e, s, rows, hp, s.split_len, s.batch_work);
+        }
         // then the refinement's restage threshold on the chosen schedule
         if (!rc && e->solve == SOLVE_CHOL_REFINE && e->cfg.refine_steps > 0) {
             const int r0 = s.restage_max;
@@ -1069,17 +1115,8 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             float bestt = 1e30f;
             for (int rm : {r0, 0, 32, 64, 128, 512, 1024, 1 << 30}) {
                 s.restage_max = rm;
-                float t = 1e30f;
-                for (int rep = 0; rep < 3 && !rc; rep++) {
-                    if ((rc = restore())) break;
-                    HIPCHK(e, hipEventRecord(ev[0], e->stream));
-                    if ((rc = update_side(e, sd))) break;
-                    HIPCHK(e, hipEventRecord(ev[1], e->stream));
-                    HIPCHK(e, hipEventSynchronize(ev[1]));
-                    float ms = 0.f;
-                    HIPCHK(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
-                    if (rep > 0) t = std::min(t, ms);
-                }
+                float t;
+                if ((rc = trial(t))) break;
                 if (t < bestt) {
                     bestt = t;
                     bestr = rm;
@@ -1095,17 +1132,8 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             float bestvt = 1e30f;
             for (int v : {v0, 1 - v0}) {
                 s.variant = v;
-                float t = 1e30f;
-                for (int rep = 0; rep < 3 && !rc; rep++) {
-                    if ((rc = restore())) break;
-                    HIPCHK(e, hipEventRecord(ev[0], e->stream));
-                    if ((rc = update_side(e, sd))) break;
-                    HIPCHK(e, hipEventRecord(ev[1], e->stream));
-                    HIPCHK(e, hipEventSynchronize(ev[1]));
-                    float ms = 0.f;
-                    HIPCHK(e, hipEventElapsedTime(&ms, ev[0], ev[1]));
-                    if (rep > 0) t = std::min(t, ms);
-                }
+                float t;
+                if ((rc = trial(t))) break;
                 if (t < bestvt) {
                     bestvt = t;
                     bestv = v;
@@ -1118,6 +1146,7 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
         if (!rc) HIPCHK(e, hipStreamSynchronize(e->stream));
         res.restage_max[sd] = s.restage_max;
         res.kernel_variant[sd] = s.variant;
+        res.grid[sd] = s.grid;
         dev_free(e, xsave);
         dev_free(e, xssave);
         dev_free(e, fsave);

@@ -145,13 +145,17 @@ class Engine:
         self._check(lib.als_autotune(self.h, sides, C.byref(r)))
         return {"split_len": list(r.split_len), "batch_work": list(r.batch_work),
                 "default_ms": list(r.default_ms), "best_ms": list(r.best_ms), "candidates": list(r.candidates),
-                "restage_max": list(r.restage_max), "kernel_variant": list(r.kernel_variant)}
+                "restage_max": list(r.restage_max), "kernel_variant": list(r.kernel_variant),
+                "grid": list(r.grid)}
 
     def set_schedule(self, side: int, split_len: int = 0, batch_work: int = 0, restage_max: int = -1) -> None:
         self._check(lib.als_set_schedule(self.h, side, split_len, batch_work, restage_max))
 
     def set_kernel_variant(self, side: int, variant: int) -> None:
         self._check(lib.als_set_kernel_variant(self.h, side, variant))
+
+    def set_grid(self, side: int, grid: int) -> None:
+        self._check(lib.als_set_grid(self.h, side, grid))
 
     def device_factors(self, side: int) -> int:
         p = C.c_void_p()

@@ -194,8 +194,10 @@ int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_
 /* Autotuner (no reference counterpart: the reference launches one block per
  * row with a fixed shape). For each side selected in `sides` (bit 0 users,
  * bit 1 items) times one half-sweep per candidate schedule, split length x
- * batch size around the cost-model defaults, then (with refinement) the
- * residual's restage threshold, on the engine's data, and keeps the fastest. Factors and the failure count are left as they were.
+ * batch size around the cost-model defaults, then the update kernel's grid,
+ * (with refinement) the residual's restage threshold and the kernel variant,
+ * on the engine's data, and keeps the fastest. Factors and the failure count
+ * are left as they were.
  * Call after als_set_train; results are identical in exact arithmetic for
  * every schedule (split rows sum their parts in a fixed order). */
 typedef struct {
@@ -207,6 +209,9 @@ typedef struct {
                                             copy of the tiles (tuned with refine_steps > 0) */
     int kernel_variant[2];               /* update kernel variant chosen (0: default, 1: one
                                             wave per SIMD; tried where it is built) */
+    int grid[2];                         /* persistent waves of the default update kernel
+                                            chosen (the occupancy limit or up to two waves
+                                            per CU fewer; als_set_grid) */
 } als_tune_result;
 int als_autotune(als_engine *e, int sides, als_tune_result *out);
 /* Applies one side's schedule directly (e.g. a stored als_autotune result):
@@ -218,6 +223,10 @@ int als_set_schedule(als_engine *e, int side, int64_t split_len, int64_t batch_w
  * (warm-started refinement at 7 to 9 tiles; ALS_E_UNSUPPORTED elsewhere). The
  * results are identical; als_autotune times both. No reference counterpart. */
 int als_set_kernel_variant(als_engine *e, int side, int variant);
+/* Persistent waves of one side's default update kernel, 1 .. als_info.grid's
+ * value at als_create (the occupancy limit); the schedule is rebuilt for
+ * them. Results are identical for every grid. No reference counterpart. */
+int als_set_grid(als_engine *e, int side, int grid);
 
 /* Model groups (no reference counterpart; the reference trains one model per
  * run): up to 4 engines over the same rating matrix (same m, n, nnz and

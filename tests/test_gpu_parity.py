@@ -365,6 +365,36 @@ def test_schedule_invariance(toy):
     assert np.array_equal(r1, r2)
 
 
+def test_grid_invariance():
+    """The update kernel's grid (persistent waves, set per side) changes which
+    wave takes which row, never the result: bit-identical factors at the
+    occupancy limit, three quarters of it and an odd small grid, with split
+    rows (split_len 300) and light rows through the dual system. Grids
+    outside 1 .. the limit are refused."""
+    d = ca.scaled(ca.netflix_shape(), 0.01).generate()
+    out = []
+    for frac in (None, 0.75, "odd"):
+        cfg = E.make_config(100, 0.05, 2, E.SolveType.CHOL, E.Prec.FP16, refine=1, refine_warm=True, split_len=300)
+        with E.Engine(cfg, d) as e:
+            gmax = e.info().grid[0]
+            if frac is not None:
+                g = int(gmax * frac) if frac != "odd" else 37
+                for side in (0, 1):
+                    e.set_grid(side, g)
+                assert list(e.info().grid) == [g, g]
+            with pytest.raises(ca.AlsError):
+                e.set_grid(0, gmax + 1)
+            with pytest.raises(ca.AlsError):
+                e.set_grid(1, 0)
+            e.init_factors()
+            e.train(2, with_rmse=False)
+            info = e.info()
+            assert info.failures == 0 and info.nsplit_rows[1] > 0 and info.dual_rows[0] > 0
+            out.append((e.get_factors(0), e.get_factors(1)))
+    for U, V in out[1:]:
+        assert np.array_equal(U, out[0][0]) and np.array_equal(V, out[0][1])
+
+
 def _run_split(d, f, prec, split_len, sweeps=2):
     cfg = E.make_config(f, 0.05, sweeps, E.SolveType.CHOL, prec, refine=1 if prec != E.Prec.FP32 else 0,
                         split_len=split_len)
@@ -420,7 +450,8 @@ def test_split_rows_are_deterministic(toy):
 
 
 def test_autotune_keeps_state_and_results():
-    """The autotuner times candidate schedules on the engine's own data: the
+    """The autotuner times candidate schedules (split length x batch size,
+    then the grid) on the engine's own data: the
     factors it leaves behind are the ones it found, the chosen schedule is
     the fastest candidate (the default is one of them), and a sweep on the
     tuned schedule matches a sweep on the default one."""
@@ -440,6 +471,8 @@ def test_autotune_keeps_state_and_results():
             assert r["best_ms"][sd] <= r["default_ms"][sd]
             assert r["split_len"][sd] >= 256 and r["batch_work"][sd] >= 1
         assert list(e1.info().split_len) == r["split_len"]
+        gmax = e0.info().grid[0]
+        assert list(e1.info().grid) == r["grid"] and all(3 * gmax <= 4 * g <= 4 * gmax for g in r["grid"])
         # The tuned restage threshold moves rows between the two refinement
         # residuals, which converge to different systems (restaged from the
         # FP32 rows: the exact one; the FP32-accumulated 16-bit tiles: the
