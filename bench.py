@@ -312,8 +312,8 @@ def main() -> int:
         fp32 = {"sweeps_per_s": a.steps / F["dt"], "users_ms": F["users_ms"], "items_ms": F["items_ms"],
                 "test_rmse": F["rmse"], "failures": int(F["info"].failures), "vs_oracle": F["vs_oracle"],
                 "solve": ("fp32 MFMA Gram, fp32 Cholesky, no refinement"
-                          + (f"; rows under {F['info'].dual_max} ratings through their n x n dual system"
-                             if F["info"].dual_max else ""))}
+                          + (f"; rows under {list(F['info'].dual_limit)} ratings (users, items) through their "
+                             f"n x n dual system" if F["info"].dual_max else ""))}
         cg_cfg = E.make_config(f, lam, a.steps, E.SolveType.CUMF_ALS_CG_FP32, E.Prec.FP32, cg_iters=6, cg_tol=1e-4,
                                use_graph=True)
         G = run(cg_cfg, "fp32-cg6", "cg6")
@@ -353,7 +353,7 @@ def main() -> int:
                    "gram": f"{a.prec} MFMA, fp32 accumulate",
                    "solve": (f"fp32 Cholesky + {a.refine} fp64-residual refinement step(s)"
                              + (", the first from the previous iterate" if a.refine_warm and a.refine else "")
-                             + (f"; rows under {info_m.dual_max} ratings ({info_m.dual_rows[0]} users, "
+                             + (f"; rows under {list(info_m.dual_limit)} ratings (users, items; {info_m.dual_rows[0]} users, "
                                 f"{info_m.dual_rows[1]} items) through their n x n dual system (fp32 "
                                 f"Cholesky, no refinement step)"
                                 if info_m.dual_max else "")),

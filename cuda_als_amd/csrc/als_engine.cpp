@@ -39,8 +39,11 @@ struct Side {
     int variant = 0;               // update kernel variant (1: one wave per SIMD, where built)
     int restage_max = 0;           // refinement: restage rows up to this long (else tile copy)
     float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
-    // Light rows (fewer than dual_max ratings, and empty rows) taken by the
-    // dual kernel: their own schedule, whole rows only.
+    // Light rows (fewer than dual_limit ratings, and empty rows) taken by the
+    // dual kernel: their own schedule, whole rows only. dual_limit is at most
+    // the engine's dual_max (the tile count the dual kernel is built for);
+    // the autotuner picks it per side.
+    int dual_limit = 0;
     int4 *dunits = nullptr;
     int *dbatch = nullptr;
     int dnbatches = 0, dnunits = 0;
@@ -183,11 +186,10 @@ int build_side(als_engine *e, Side &s, int64_t rows, const int *ptr, int64_t spl
     // With a dual kernel, the light rows get a schedule of their own.
     std::vector<int> heavy, light;
     const bool ranged = rb > 0 || re < rows;
-    if (e->dual_max > 0 || ranged) {
-        for (int64_t r = rb; r < re; r++)
-            (e->dual_max > 0 && ptr[r + 1] - ptr[r] < e->dual_max ? light : heavy).push_back((int)r);
+    const bool dual = s.dual_limit > 0;
+    if (dual || ranged) {
+        for (int64_t r = rb; r < re; r++) (dual && ptr[r + 1] - ptr[r] < s.dual_limit ? light : heavy).push_back((int)r);
     }
-    const bool dual = e->dual_max > 0;
     Schedule sc = build_schedule(rows, ptr, (int)split_len, (int)batch_work, s.grid, e->prec,
                                  dual || ranged ? &heavy : nullptr);
     Schedule sd;
@@ -487,6 +489,7 @@ int als_create(const als_config *cfg, als_engine **out) {
             rc = set_err(e, ALS_E_HIP, "occupancy query failed (dual kernel)");
         e->dual_grid = std::max(1, dpc) * e->num_cu;
         e->dual_max = dual_max_ratings(e->T, e->prec);
+        e->side[0].dual_limit = e->side[1].dual_limit = e->dual_max;
     }
     if (!rc && e->solve == SOLVE_CHOL_REFINE)
         rc = dev_alloc(e, &e->scratch, (size_t)e->grid_max * (e->T * (e->T + 1) / 2) * 256);
@@ -793,6 +796,8 @@ int als_get_info(als_engine *e, als_info *o) {
     o->nnz = e->nnz;
     o->n_test = e->n_test;
     o->dual_max = e->dual_max;
+    o->dual_limit[0] = e->side[0].dual_limit;
+    o->dual_limit[1] = e->side[1].dual_limit;
     o->dual_rows[0] = e->side[0].dnunits;
     o->dual_rows[1] = e->side[1].dnunits;
     for (int sd = 0; sd < 2; sd++) {
@@ -1002,12 +1007,29 @@ extern "C" int als_set_grid(als_engine *e, int sd, int grid) {
     return build_side(e, s, s.rows, e->hptr[sd].data(), s.split_len, s.batch_work);
 }
 
+// Rows of one side with fewer than `limit` ratings (and empty rows) take the
+// dual system; 0 sends every row through the f x f system. At most the
+// engine's dual_max.
+extern "C" int als_set_dual_limit(als_engine *e, int sd, int limit) {
+    if (!e) return ALS_E_INVALID;
+    if (!e->have_train) return set_err(e, ALS_E_STATE, "set the train data first");
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (limit < 0 || limit > e->dual_max)
+        return set_err(e, ALS_E_INVALID, "dual limit must be between 0 and " + std::to_string(e->dual_max));
+    drop_graph(e);
+    Side &s = e->side[sd];
+    s.dual_limit = limit;
+    return build_side(e, s, s.rows, e->hptr[sd].data(), s.split_len, s.batch_work);
+}
+
 // Autotuner: times one half-sweep per candidate schedule (split length x
 // batch size, around the cost-model defaults) on the engine's own data and
 // keeps the fastest; then the update kernel's grid (fewer persistent waves
 // than the occupancy limit can gather faster: the items half of the
 // netflix-shaped workload runs 4% faster at six waves per CU than at
-// eight), the restage threshold and the kernel variant. The side's factors and the failure count are restored
+// eight), the shape of the light rows' systems (the rating count below which
+// a row solves its n x n dual system in ceil((n+1)/16) tiles instead of the
+// f x f one in T tiles), the restage threshold and the kernel variant. The side's factors and the failure count are restored
 // after every trial, so tuning leaves the model state unchanged.
 extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
     if (!e) return ALS_E_INVALID;
@@ -1027,6 +1049,7 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             res.restage_max[sd] = s.restage_max;
             res.kernel_variant[sd] = s.variant;
             res.grid[sd] = s.grid;
+            res.dual_limit[sd] = s.dual_limit;
             continue;
         }
         const int64_t rows = s.rows;
@@ -1108,6 +1131,23 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             s.grid = bestg;
             if (!rc) rc = build_side(e, s, rows, hp, s.split_len, s.batch_work);
         }
+        // then the dual limit: the built maximum and one or two tiles fewer
+        if (!rc && e->dual_max > 0) {
+            const int d0 = s.dual_limit;
+            int bestd = d0;
+            for (int dl : {e->dual_max, e->dual_max - 16, e->dual_max - 32}) {
+                if (dl < 16 || dl == d0) continue;
+                s.dual_limit = dl;
+                float t;
+                if ((rc = build_side(e, s, rows, hp, s.split_len, s.batch_work)) || (rc = trial(t))) break;
+                if (t < best) {
+                    best = t;
+                    bestd = dl;
+                }
+            }
+            s.dual_limit = bestd;
+            if (!rc) rc = build_side(e, s, rows, hp, s.split_len, s.batch_work);
+        }
         // then the refinement's restage threshold on the chosen schedule
         if (!rc && e->solve == SOLVE_CHOL_REFINE && e->cfg.refine_steps > 0) {
             const int r0 = s.restage_max;
@@ -1147,6 +1187,7 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
         res.restage_max[sd] = s.restage_max;
         res.kernel_variant[sd] = s.variant;
         res.grid[sd] = s.grid;
+        res.dual_limit[sd] = s.dual_limit;
         dev_free(e, xsave);
         dev_free(e, xssave);
         dev_free(e, fsave);

@@ -98,10 +98,12 @@ typedef struct {
     int64_t split_len[2];/* ratings per part of a split row */
     int64_t failures;    /* rows whose factorisation broke down (x set to 0) */
     int64_t m, n, nnz, n_test;
-    int dual_max;        /* rows with fewer ratings take the dual system (0: none) */
+    int dual_max;        /* the dual system's largest rating count + 1 (0: no dual kernel) */
     int dual_rows[2];    /* rows of each side the dual system takes */
     int kernel_variant[2]; /* update kernel variant per side (als_set_kernel_variant) */
     int grid_variant1;   /* persistent waves of the one-wave-per-SIMD variant (0: not built) */
+    int dual_limit[2];   /* rows with fewer ratings take the dual system, per side
+                            (<= dual_max; als_set_dual_limit, autotuned) */
 } als_info;
 
 int als_config_init(als_config *cfg);
@@ -195,6 +197,7 @@ int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_
  * row with a fixed shape). For each side selected in `sides` (bit 0 users,
  * bit 1 items) times one half-sweep per candidate schedule, split length x
  * batch size around the cost-model defaults, then the update kernel's grid,
+ * the dual-system limit (which rows solve n x n instead of f x f systems),
  * (with refinement) the residual's restage threshold and the kernel variant,
  * on the engine's data, and keeps the fastest. Factors and the failure count
  * are left as they were.
@@ -212,6 +215,9 @@ typedef struct {
     int grid[2];                         /* persistent waves of the default update kernel
                                             chosen (the occupancy limit or up to two waves
                                             per CU fewer; als_set_grid) */
+    int dual_limit[2];                   /* rating count below which rows take the dual
+                                            system (dual_max or one or two tiles fewer;
+                                            als_set_dual_limit) */
 } als_tune_result;
 int als_autotune(als_engine *e, int sides, als_tune_result *out);
 /* Applies one side's schedule directly (e.g. a stored als_autotune result):
@@ -227,6 +233,11 @@ int als_set_kernel_variant(als_engine *e, int side, int variant);
  * value at als_create (the occupancy limit); the schedule is rebuilt for
  * them. Results are identical for every grid. No reference counterpart. */
 int als_set_grid(als_engine *e, int side, int grid);
+/* Rows of one side with fewer than `limit` ratings (and empty rows) solve
+ * their n x n dual system; 0 sends every row through the f x f system. 0 ..
+ * als_info.dual_max. Both systems have the same solution in exact
+ * arithmetic. No reference counterpart. */
+int als_set_dual_limit(als_engine *e, int side, int limit);
 
 /* Model groups (no reference counterpart; the reference trains one model per
  * run): up to 4 engines over the same rating matrix (same m, n, nnz and

@@ -666,6 +666,48 @@ def test_dual_system_tile_boundaries(f, prec, refine, warm, oprec, tol):
     assert err.max() <= tol
 
 
+def test_dual_limit_moves_rows_between_systems():
+    """The per-side dual limit decides which rows solve the n x n dual system
+    and which the f x f one: at limits 80 (the built maximum at f = 100), 48,
+    16 and 0 (none), one warm-started users half-sweep from random factors
+    takes exactly the rows below the limit (and the empty ones) through the
+    dual kernel, and every row matches the FP64 solve of the FP16-rounded
+    system within 1e-5 relative to max(1, |x|). Limits above the maximum
+    are refused."""
+    rng = np.random.default_rng(29)
+    f = 100
+    counts = np.repeat([0, 1, 15, 16, 17, 40, 47, 48, 49, 63, 64, 79, 80, 81, 150], 3)
+    m, n_items = len(counts), 400
+    cols = np.concatenate([np.sort(rng.choice(n_items, c, replace=False)) for c in counts]).astype(np.int32)
+    ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    vals = rng.integers(1, 6, cols.size).astype(np.float32)
+    rows = np.repeat(np.arange(m), counts)
+    d = ca.from_arrays(m, n_items, ptr, cols, vals, rows[:5], cols[:5], vals[:5])
+    U0 = (0.3 * rng.standard_normal((m, f))).astype(np.float32)
+    V0 = (0.3 * rng.standard_normal((n_items, f))).astype(np.float32)
+    Un, _ = O.update(O.config(f, 0.05, O.SOLVE_CHOL, gram_prec=O.PREC_FP16), d.csr_ptr, d.csr_idx, d.csr_val, V0,
+                     U0.astype(np.float64))
+    cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.FP16, refine=1, refine_warm=True)
+    with E.Engine(cfg, d) as e:
+        info = e.info()
+        assert info.dual_max == 80 and list(info.dual_limit) == [80, 80]
+        with pytest.raises(ca.AlsError):
+            e.set_dual_limit(0, 81)
+        for limit in (80, 48, 16, 0):
+            e.set_dual_limit(0, limit)
+            e.set_factors(0, U0)
+            e.set_factors(1, V0)
+            e.update(ca.ALS_SIDE_USERS)
+            Ug = e.get_factors(0)
+            info = e.info()
+            assert info.dual_limit[0] == limit and info.failures == 0
+            assert info.dual_rows[0] == (int(np.sum(counts < limit)) if limit else 0)
+            err = np.abs(Ug - Un).max(axis=1) / np.maximum(1.0, np.abs(Un).max(axis=1))
+            print(limit, err.max())
+            assert not Ug[counts == 0].any()
+            assert err.max() <= 1e-5
+
+
 @pytest.mark.parametrize("f", [16, 63, 100, 127])
 @pytest.mark.parametrize("prec,oprec", [(E.Prec.FP16, O.PREC_FP16), (E.Prec.BF16, O.PREC_BF16)])
 @pytest.mark.parametrize("split_len", [0, 40])
