@@ -615,15 +615,29 @@ __device__ __forceinline__ void diag_factor(f32x4 &d, int jend) {
 
 // A tile's four registers as FP16 hi and lo parts, x = hi + lo + O(2^-22 x):
 // the operand of a 16x16x16 FP16 MFMA in the register layout of the FP32
-// 16x16x4 form (lane (q, c) holds k = 4q .. 4q+3 of row c).
+// 16x16x4 form (lane (q, c) holds k = 4q .. 4q+3 of row c). hi = fp16(x)
+// and lo = fp16(x - hi), both rounded to nearest; x - hi is exact in FP32.
+// Two instructions per value: packed conversions (two values each) and
+// v_fma_mix_f32, which reads hi as FP16 and forms x - hi in one step.
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <int HALF>
+__device__ __forceinline__ float minus_half(f16x2 h, float x) {  // x - h[HALF], exact
+    float r;
+    if constexpr (HALF == 0)
+        asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+    else
+        asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+    return r;
+}
 __device__ __forceinline__ void split16(const f32x4 &x, f16x4 &hi, f16x4 &lo) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const _Float16 h = (_Float16)x[r];
-        hi[r] = h;
-        lo[r] = (_Float16)(x[r] - (float)h);
-    }
+    const f16x2 h01 = __builtin_convertvector(f32x2{x[0], x[1]}, f16x2);
+    const f16x2 h23 = __builtin_convertvector(f32x2{x[2], x[3]}, f16x2);
+    const f16x2 l01 = __builtin_convertvector(f32x2{minus_half<0>(h01, x[0]), minus_half<1>(h01, x[1])}, f16x2);
+    const f16x2 l23 = __builtin_convertvector(f32x2{minus_half<0>(h23, x[2]), minus_half<1>(h23, x[3])}, f16x2);
+    hi = __builtin_shufflevector(h01, h23, 0, 1, 2, 3);
+    lo = __builtin_shufflevector(l01, l23, 0, 1, 2, 3);
 }
 
 // Blocked right-looking Cholesky, panel K:
