@@ -1112,14 +1112,14 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             }
         }
         if (!rc) rc = build_side(e, s, rows, hp, cand[besti].first, cand[besti].second);
-        // then the grid: the occupancy limit and one or two waves per CU fewer
-        // (down to three quarters of it), the schedule rebuilt for each
+        // then the grid: the occupancy limit and up to half as many waves per
+        // CU, the schedule rebuilt for each
         if (!rc) {
             const int pc = e->grid_max / e->num_cu, g0 = s.grid;
             int bestg = g0;
-            for (int k = 1; k <= 2 && !rc; k++) {
+            for (int k = 1; k <= 4 && !rc; k++) {
                 const int g = (pc - k) * e->num_cu;
-                if (4 * (pc - k) < 3 * pc || pc - k < 1) break;
+                if (2 * (pc - k) < pc || pc - k < 1) break;
                 s.grid = g;
                 float t;
                 if ((rc = build_side(e, s, rows, hp, s.split_len, s.batch_work)) || (rc = trial(t))) break;

@@ -213,8 +213,8 @@ typedef struct {
     int kernel_variant[2];               /* update kernel variant chosen (0: default, 1: one
                                             wave per SIMD; tried where it is built) */
     int grid[2];                         /* persistent waves of the default update kernel
-                                            chosen (the occupancy limit or up to two waves
-                                            per CU fewer; als_set_grid) */
+                                            chosen (the occupancy limit or down to half of
+                                            it, in steps of one wave per CU; als_set_grid) */
     int dual_limit[2];                   /* rating count below which rows take the dual
                                             system (dual_max or one or two tiles fewer;
                                             als_set_dual_limit) */
