@@ -643,12 +643,12 @@ __device__ __forceinline__ void split16(const f32x4 &x, f16x4 &hi, f16x4 &lo) {
 // Blocked right-looking Cholesky, panel K:
 //   diagonal block -> W_KK = L_KK^-1 (diag_factor)
 //   panel          L_IK^T = W_KK A_IK^T       (4 MFMAs per tile)
-//   trailing       A_IJ  -= L_IK L_JK^T        (4 FP32 MFMAs per tile; with
-//                  SPLIT three FP16 16x16x16 MFMAs on the hi/lo parts of
-//                  the panel tiles, hi hi + hi lo + lo hi, each FP32 product
-//                  kept to ~2^-20 relative at an eighth of the MFMA time;
-//                  the entries of L are bounded by sqrt(A_ii), well inside
-//                  FP16's range)
+//   trailing       A_IJ  -= L_IK L_JK^T        (4 FP32 MFMAs per tile, 128
+//                  cycles; with SPLIT two FP16 16x16x32 MFMAs on the hi/lo
+//                  parts of the panel tiles, 32 cycles, each FP32 product
+//                  kept to ~2^-22 relative; the entries of L are bounded by
+//                  sqrt(A_ii), inside FP16's range for the ratings
+//                  als_set_train accepts)
 // The last block holds row f (the augmented row, never a pivot): yl gets
 // its reduced entries (row form) just before that block is factored.
 template <int T, int K, int NT, bool SPLIT>
@@ -672,18 +672,29 @@ __device__ __forceinline__ void chol_panels(f32x4 (&a)[NT], int f, f32x4 &yl) {
                 a[tile_id(I, K)] = nw;
             }
             if constexpr (SPLIT) {
-                f16x4 hi[T], lo[T];
+                // Each panel tile as [hi | lo], the 8 halves a lane feeds a
+                // 16x16x32 MFMA (the contraction pairs position j of lane
+                // (q, c) in one operand with position j of lane (q, c') in the
+                // other). Against [hi_J | hi_J] it gives hi_J hi_I + hi_J lo_I,
+                // against [lo_J | lo_J] lo_J hi_I + lo_J lo_I: the whole
+                // product in two MFMAs of 16 cycles (three 16x16x16 ones
+                // took 48).
+                f16x8 hl[T];
 #pragma unroll
-                for (int I = K + 1; I < T; ++I) split16(a[tile_id(I, K)], hi[I], lo[I]);
+                for (int I = K + 1; I < T; ++I) {
+                    f16x4 h, l;
+                    split16(a[tile_id(I, K)], h, l);
+                    hl[I] = __builtin_shufflevector(h, l, 0, 1, 2, 3, 4, 5, 6, 7);
+                }
 #pragma unroll
                 for (int J = K + 1; J < T; ++J) {
-                    const f16x4 nh = -hi[J], nl = -lo[J];
+                    const f16x8 nhh = -__builtin_shufflevector(hl[J], hl[J], 0, 1, 2, 3, 0, 1, 2, 3);
+                    const f16x8 nll = -__builtin_shufflevector(hl[J], hl[J], 4, 5, 6, 7, 4, 5, 6, 7);
 #pragma unroll
                     for (int I = J; I < T; ++I) {
                         f32x4 t = a[tile_id(I, J)];
-                        t = __builtin_amdgcn_mfma_f32_16x16x16f16(nh, hi[I], t, 0, 0, 0);
-                        t = __builtin_amdgcn_mfma_f32_16x16x16f16(nh, lo[I], t, 0, 0, 0);
-                        t = __builtin_amdgcn_mfma_f32_16x16x16f16(nl, hi[I], t, 0, 0, 0);
+                        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(nhh, hl[I], t, 0, 0, 0);
+                        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(nll, hl[I], t, 0, 0, 0);
                         a[tile_id(I, J)] = t;
                     }
                 }
