@@ -39,6 +39,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 __host__ __device__ constexpr int tile_id(int I, int J) { return I * (I + 1) / 2 + J; }
 
@@ -115,6 +116,12 @@ template <int T, int E> struct Geo {
     // instruction share four).
     __host__ __device__ static constexpr int yt_row(int j) { return j + (j >> 3); }
     static constexpr int YR = (T & 1) ? FP : FP + 16;  // FP32 stage row stride (floats)
+    // Rating-major 16-bit stage of the pipelined Gram (row k = rating k of
+    // the chunk, column j = factor j): row stride in halves. A transposing
+    // operand read takes 32 bytes from each of four consecutive rows; with
+    // 2 * RS16 bytes an odd multiple of 32 modulo 256, or T = 2 mod 4, the
+    // four land on distinct LDS banks.
+    static constexpr int RS16 = (T % 4 == 0) ? FP + 16 : FP;
     // Rows with fewer than 16 * DUAL_TD ratings take the dual system (T >= 5).
     static constexpr int DUAL_TD = T >= 5 ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
     static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : yt_row(FP) * YT_H * 2;
@@ -234,27 +241,40 @@ __device__ __forceinline__ void stage_chunk(const UpdateArgs &a, int64_t rs, int
 // 16-bit Gram of one row when one gather task per lane covers a chunk
 // (4 * nseg <= 64, f <= 128), software-pipelined: the gathers of chunk k+1
 // are issued before the MFMAs of chunk k and land in registers while they
-// run, and the indices and ratings of chunk k+2 are loaded meanwhile. The
-// stage, the task layout and the MFMA order are those of the plain loop,
-// so the sums are the same bit for bit. Stage rows 8s + e (e < 8) sit at
-// 9s + e and rows 16I + c at 18I + yt_row(c), so each lane addresses its
-// writes and operand reads from one base with constant offsets.
+// run, and the indices and ratings of chunk k+2 are loaded meanwhile.
+// Task (g, s) of a lane: segment s (factors 8s .. 8s+7) of ratings 8g ..
+// 8g+7, eight 16-byte loads written as they arrive to a rating-major stage
+// Yr[rating][factor]. The MFMA operands come out of it transposed: for
+// column block I, lane (q, c) needs A[c][k] = Yr[k][16I + c] for k = 8q ..
+// 8q+7, and two ds_read_b64_tr_b16 give it: within each 16-lane group,
+// lane i supplies the address of 4 halves, Yr[8q + 4h + i/4][16I + 4(i%4)
+// .. +3], and lane j receives element j + 16t of the group's 64, which is
+// Yr[8q + 4h + t][16I + j], t = 0..3.
 template <int T, int E>
 __device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs, int n,
                                                  f32x4 (&acc)[Geo<T, E>::NT], char *lds) {
     using G = Geo<T, E>;
+    constexpr int RS = G::RS16;
+    typedef short v4i16 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) v4i16 lds_v4i16;
     const int l = lane();
     const int f = a.f;
     int *ibuf = reinterpret_cast<int *>(lds);
     float *rbuf = reinterpret_cast<float *>(lds) + 32;
-    uint16_t *Yt = reinterpret_cast<uint16_t *>(lds + G::MISC_FLOATS * 4);
+    uint16_t *Yr = reinterpret_cast<uint16_t *>(lds + G::MISC_FLOATS * 4);
     const int nseg = (f + 7) >> 3;
     const bool active = l < 4 * nseg;  // task (g, s) of lane l
     const int g = l / nseg, s = l - g * nseg;
-    uint16_t *wbase = Yt + 9 * s * G::YT_H + 8 * g;
-    const uint16_t *rbase = Yt + G::yt_row(l & 15) * G::YT_H + 8 * (l >> 4);
+    uint16_t *wbase = Yr + 8 * g * RS + 8 * s;
+    const uint16_t *rbase = Yr + (8 * (l >> 4) + ((l & 15) >> 2)) * RS + 4 * (l & 3);
     const int64_t soff = 8 * s;
     if (n <= 0) return;
+    // columns no gather writes (8 nseg .. 16T): zero, but for the rating
+    // column f, which each chunk writes (when f % 8 != 0 it lies in the last
+    // gathered segment, whose padding halves are zero in the shadow rows)
+    // (in 16-byte pieces: piece e is row e % 32, columns 8 nseg + 8 (e / 32))
+    for (int e = l; e < 4 * (G::FP - 8 * nseg); e += WAVE)
+        *reinterpret_cast<u32x4 *>(Yr + (e & 31) * RS + 8 * nseg + 8 * (e >> 5)) = u32x4{0u, 0u, 0u, 0u};
     // Raw indices and ratings of the chunk at k0, loaded by every lane (lane
     // l and l + 32 alike) at positions clamped into the row, so the loads
     // are unconditional; the positions past the row's end are masked when
@@ -296,27 +316,22 @@ __device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs
     for (int k0 = 0; k0 < n; k0 += CHUNK) {
         if (active) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                u32x4 lo, hi;
-#pragma unroll
-                for (int w = 0; w < 4; ++w) {
-                    const uint32_t x0 = d[2 * w][v], x1 = d[2 * w + 1][v];
-                    lo[w] = __builtin_amdgcn_perm(x1, x0, 0x05040100u);  // low halves of x0, x1
-                    hi[w] = __builtin_amdgcn_perm(x1, x0, 0x07060302u);  // high halves
-                }
-                *reinterpret_cast<u32x4 *>(wbase + (2 * v) * G::YT_H) = lo;
-                *reinterpret_cast<u32x4 *>(wbase + (2 * v + 1) * G::YT_H) = hi;
-            }
+            for (int e = 0; e < 8; ++e) *reinterpret_cast<u32x4 *>(wbase + e * RS) = d[e];
         }
-        if (const int lh = lane_here(); lh < CHUNK) Yt[G::yt_row(f) * G::YT_H + lh] = to16<E>(rbuf[lh]);
+        if (const int lh = lane_here(); lh < CHUNK) Yr[lh * RS + f] = to16<E>(rbuf[lh]);
         wave_sync();
         u32x4 F[T];
 #pragma unroll
-        for (int I = 0; I < T; ++I) F[I] = *reinterpret_cast<const u32x4 *>(rbase + 18 * I * G::YT_H);
+        for (int I = 0; I < T; ++I) {
+            const v4i16 t0 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4i16 *)(rbase + 16 * I));
+            const v4i16 t1 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4i16 *)(rbase + 4 * RS + 16 * I));
+            const u32x2 w0 = __builtin_bit_cast(u32x2, t0), w1 = __builtin_bit_cast(u32x2, t1);
+            F[I] = u32x4{w0[0], w0[1], w1[0], w1[1]};
+        }
         if (k0 + CHUNK < n) {
             // The operand reads above are issued before the next chunk's
-            // transposes write the stage, and LDS keeps a wave's operations
-            // in order; ibuf and rbuf are read before they are rewritten.
+            // gathers write the stage, and LDS keeps a wave's operations in
+            // order; ibuf and rbuf are read before they are rewritten.
             gather(k0 + CHUNK, nid, nval);
             load_ids(k0 + 2 * CHUNK, nid, nval);
         }
@@ -345,7 +360,6 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
     char *stage = lds + G::MISC_FLOATS * 4;
 #pragma unroll
     for (int t = 0; t < G::NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    stage_clear<T, E>(a.f, stage);
     if constexpr (E != PREC_FP32 && T <= 9) {
         // f <= 128: at most 16 segments, one task per lane (every f of
         // T <= 8; f = 128 of T = 9, whose rating column is the 129th)
@@ -354,6 +368,7 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
             return;
         }
     }
+    stage_clear<T, E>(a.f, stage);
     for (int k0 = 0; k0 < n; k0 += CHUNK) {
         stage_chunk<T, E>(a, rs, k0, min(CHUNK, n - k0), lds);
         if constexpr (E == PREC_FP32) {
