@@ -268,6 +268,10 @@ __device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs
     uint16_t *wbase = Yr + 8 * g * RS + 8 * s;
     const uint16_t *rbase = Yr + (8 * (l >> 4) + ((l & 15) >> 2)) * RS + 4 * (l & 3);
     const int64_t soff = 8 * s;
+    const int soff2 = 16 * s;  // the same in bytes
+    // the shadow as a buffer (raw, byte offsets, zeros past ys_bytes)
+    const __amdgpu_buffer_rsrc_t ysr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(a.Ys), (short)0, a.ys_bytes, 0x00020000);
     if (n <= 0) return;
     // columns no gather writes (8 nseg .. 16T): zero, but for the rating
     // column f, which each chunk writes (when f % 8 != 0 it lies in the last
@@ -301,10 +305,20 @@ __device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs
             const int4 i0 = *reinterpret_cast<const int4 *>(ibuf + 8 * gi);
             const int4 i1 = *reinterpret_cast<const int4 *>(ibuf + 8 * gi + 4);
             const int ids[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+            if (a.ys_bytes > 0) {
+                // Buffer loads with a 32-bit byte offset: a slot past the
+                // row's end (id -1) has an offset of 2^32 - 2 FP + 2 soff,
+                // beyond the buffer, and reads zeros.
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const uint16_t *src = (ids[e] >= 0 ? a.Ys + (int64_t)ids[e] * G::FP : a.zero16) + soff;
-                d[e] = *reinterpret_cast<const u32x4 *>(src);
+                for (int e = 0; e < 8; ++e)
+                    d[e] = __builtin_bit_cast(
+                        u32x4, __builtin_amdgcn_raw_buffer_load_b128(ysr, ids[e] * (2 * G::FP) + soff2, 0, 0));
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint16_t *src = (ids[e] >= 0 ? a.Ys + (int64_t)ids[e] * G::FP : a.zero16) + soff;
+                    d[e] = *reinterpret_cast<const u32x4 *>(src);
+                }
             }
         }
     };

@@ -270,6 +270,11 @@ UpdateArgs side_args(als_engine *e, const Side &s, const float *Y, const uint16_
     a.scratch = e->scratch;
     a.zero32 = e->zero32;
     a.zero16 = e->zero16;
+    {
+        const int64_t ny = &s == &e->side[0] ? e->n : e->m;  // rows of Y
+        const int64_t bytes = ny * e->ld * (int64_t)sizeof(uint16_t);
+        a.ys_bytes = bytes <= INT32_MAX ? (int)bytes : 0;
+    }
     return a;
 }
 

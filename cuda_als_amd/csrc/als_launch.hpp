@@ -42,6 +42,8 @@ struct UpdateArgs {
     float *scratch;             // per-wave copy of A for refinement, NT*256 floats per workgroup
     const float *zero32;        // one row of zeros (16*T floats) read by unused chunk slots
     const uint16_t *zero16;     // same, 16-bit
+    int ys_bytes;               // bytes of the 16-bit Y shadow when below 2^31 (the pipelined
+                                // Gram then gathers with 32-bit buffer offsets), else 0
 };
 
 // Several models (different f) on one rating matrix, one launch: batch b of
