@@ -1397,6 +1397,23 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
             float x[T];
 #pragma unroll
             for (int J = 0; J < T; ++J) x[J] = 0.f;
+            // Warm-started refinement of a whole row: the row's previous
+            // value x0 (its FP32 row of X, 16T floats) is copied to the LDS
+            // vector area (xs) by an LDS-DMA load issued before the Gram, so
+            // its memory latency runs under the Gram; lane l copies floats
+            // 4l .. 4l+3. Only this wave writes that area, and it waits for
+            // the copy (vmcnt) before reading it, in this same iteration.
+            // (Parts of split rows, which may leave the iteration before the
+            // solve, load x0 from X when they solve.)
+            constexpr bool WARM_KERNEL = S == SOLVE_CHOL_WARM || S == SOLVE_CHOL_REFINE;
+            const bool warm_row =
+                WARM_KERNEL && (S == SOLVE_CHOL_WARM || (a.refine_warm && a.refine > 0)) && n > 0 && w.w < 0;
+            float *xs = reinterpret_cast<float *>(lds) + 64;
+            if constexpr (WARM_KERNEL) {
+                if (warm_row && l < G::FP / 4)
+                    __builtin_amdgcn_global_load_lds(a.X + (int64_t)row * G::FP + 4 * l,
+                                                     (__attribute__((address_space(3))) void *)xs, 16, 0, 0);
+            }
             if (n > 0) {
                 gram_row<T, E>(a, w.y, w.z - w.y, acc, lds);
                 if (w.w >= 0) {
@@ -1448,9 +1465,15 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     }
                     if constexpr (S == SOLVE_CHOL_REFINE || S == SOLVE_CHOL_WARM) {
                         if (warm) {
-                            load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
-                            warm_row_f<T>(acc, f, x0, reinterpret_cast<float *>(lds) + 64,
-                                          reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4));
+                            if (warm_row) {
+                                // the copy of x0 has landed in xs (and no other
+                                // load is outstanding after a whole row's Gram)
+                                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                                load_colform<T>(xs, f, x0);
+                            } else {
+                                load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
+                            }
+                            warm_row_f<T>(acc, f, x0, xs, reinterpret_cast<float *>(lds + G::MISC_FLOATS * 4));
                         }
                     }
                     f32x4 zr[T];
@@ -1458,8 +1481,10 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     ok = chol_regs<T, G::NT, (E != PREC_FP32)>(acc, f, zr);
                     back_solve<T>(acc, f, zr, x);
                     if (warm) {
-                        // x0 again from X (not kept live across the factorisation)
-                        load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
+                        // x0 again (not kept live across the factorisation): from
+                        // xs, where the residual wrote the same values, or from X
+                        if (warm_row) load_colform<T>(xs, f, x0);
+                        else load_colform<T>(a.X + (int64_t)row * G::FP, f, x0);
 #pragma unroll
                         for (int J = 0; J < T; ++J) x[J] += x0[J];
                     }
