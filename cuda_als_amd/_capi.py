@@ -46,7 +46,7 @@ class AlsInfo(C.Structure):
                 ("split_len", C.c_int64 * 2), ("failures", C.c_int64), ("m", C.c_int64), ("n", C.c_int64),
                 ("nnz", C.c_int64), ("n_test", C.c_int64), ("dual_max", C.c_int), ("dual_rows", C.c_int * 2),
                 ("kernel_variant", C.c_int * 2), ("grid_variant1", C.c_int),
-                ("dual_limit", C.c_int * 2)]
+                ("dual_limit", C.c_int * 2), ("gram_gather", C.c_int * 2)]
 
 
 class ScheduleStats(C.Structure):
@@ -58,7 +58,7 @@ class TuneResult(C.Structure):
     _fields_ = [("split_len", C.c_int64 * 2), ("batch_work", C.c_int64 * 2), ("default_ms", C.c_float * 2),
                 ("best_ms", C.c_float * 2), ("candidates", C.c_int * 2), ("restage_max", C.c_int * 2),
                 ("kernel_variant", C.c_int * 2), ("grid", C.c_int * 2),
-                ("dual_limit", C.c_int * 2)]
+                ("dual_limit", C.c_int * 2), ("gram_gather", C.c_int * 2)]
 
 
 class SynthSpec(C.Structure):
@@ -110,6 +110,7 @@ SIGNATURES = {
     "als_set_kernel_variant": (C.c_int, [P, C.c_int, C.c_int]),
     "als_set_grid": (C.c_int, [P, C.c_int, C.c_int]),
     "als_set_dual_limit": (C.c_int, [P, C.c_int, C.c_int]),
+    "als_set_gram_gather": (C.c_int, [P, C.c_int, C.c_int]),
     "als_group_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
     "als_group_destroy": (None, [P]),
     "als_group_last_error": (C.c_char_p, [P]),

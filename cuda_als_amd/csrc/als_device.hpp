@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "als_launch.hpp"
 
 namespace als {
@@ -124,7 +126,16 @@ template <int T, int E> struct Geo {
     static constexpr int RS16 = (T % 4 == 0) ? FP + 16 : FP;
     // Rows with fewer than 16 * DUAL_TD ratings take the dual system (T >= 5).
     static constexpr int DUAL_TD = T >= 5 ? (T - 2 < ALS_DUAL_TD_MAX ? T - 2 : ALS_DUAL_TD_MAX) : 0;
-    static constexpr int GRAM_STAGE_BYTES = E == PREC_FP32 ? CHUNK * YR * 4 : yt_row(FP) * YT_H * 2;
+    // The double-buffered LDS-DMA Gram (gram_row16_dma): two rating-major
+    // buffers of CHUNK x FP halves, then a second ibuf/rbuf pair (256 bytes).
+    // Used where a row of FP halves keeps a transposing read's four rows on
+    // distinct LDS banks (T % 4 != 0) and a chunk's tasks fit a wave (T <= 8).
+    static constexpr bool DMA_GRAM = gram_dma_built(T, E);
+    static constexpr int DMA_STAGE_BYTES = 2 * CHUNK * FP * 2 + 256;
+    static constexpr int GRAM_STAGE_BYTES =
+        E == PREC_FP32 ? CHUNK * YR * 4
+                       : (DMA_GRAM && DMA_STAGE_BYTES > yt_row(FP) * YT_H * 2 ? DMA_STAGE_BYTES
+                                                                               : yt_row(FP) * YT_H * 2);
     // LDS of the update kernels (misc + Gram stage) and of the dual kernel (misc)
     // ibuf (32 ints), rbuf (32 floats), xs (FP floats), eb (32 doubles)
     static constexpr int MISC_FLOATS = 32 + 32 + FP + 64;
@@ -366,6 +377,167 @@ __device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs
     }
 }
 
+// 16-bit Gram of one row with the gathers going straight to LDS (LDS-DMA:
+// buffer_load_dwordx4 ... lds, no registers), two chunks in flight per wave.
+// Lane (g, s) of the 4 * 2T active lanes loads segment s (factors 8s ..
+// 8s+7) of ratings 4e + g, e = 0..7; load e of the wave writes its active
+// lanes' 16 bytes at 16 * lane from buffer row 4e, i.e. rating 4e + g at row
+// 4e + g, column 8s: the rating-major stage of gram_row16_piped, filled in
+// rating order, so the transposing operand reads and the MFMAs are the
+// same as there and the sums the same bit for bit. Slots past the row's end
+// are zeroed in LDS after their chunk has landed.
+// The order of vector-memory operations is fixed and counted: per chunk j,
+// I(j) = the index and rating loads (two buffer loads) and D(j) = its eight
+// LDS-DMA loads, issued as I0 I1 D0 I2 D1 I3, then per chunk k consumed:
+// D(k+2) and I(k+4), always (past the row's end they touch no memory), so
+// that when chunk k is consumed the operations after D(k) are I(k+2),
+// D(k+1) and I(k+3): s_waitcnt vmcnt(12) means D(k) has landed.
+// Before a buffer is refilled, lgkmcnt(0) makes sure its operand reads are
+// done. Index and rating loads past the row's end read its last rating.
+// The loop of gram_row16_dma, with the two stage buffers and the two index /
+// rating buffer pairs as distinct restrict pointers: the compiler's wait
+// placement for LDS-DMA then tells an operand read of one buffer from the
+// loads still landing in the other.
+template <int T, int E>
+__device__ __forceinline__ void gram_dma_loop(const UpdateArgs &a, int64_t rs, int n, f32x4 (&acc)[Geo<T, E>::NT],
+                                              uint16_t *__restrict__ buf0, uint16_t *__restrict__ buf1,
+                                              int *__restrict__ ib0, int *__restrict__ ib1,
+                                              float *__restrict__ rb0, float *__restrict__ rb1) {
+    using G = Geo<T, E>;
+    constexpr int RS = G::FP;             // stage row stride, halves
+    constexpr int NS = 2 * T;             // 16-byte segments per row
+    typedef short v4i16 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) v4i16 lds_v4i16;
+    typedef __attribute__((address_space(3))) void lds_void;
+    const int l = lane();
+    const int f = a.f;
+    const bool active = l < 4 * NS;
+    const int g = l / NS, s = l - g * NS;
+    const int soff2 = 16 * s;
+    const int roff = (8 * (l >> 4) + ((l & 15) >> 2)) * RS + 4 * (l & 3);
+    const int nch = (n + CHUNK - 1) / CHUNK;
+    const __amdgpu_buffer_rsrc_t ysr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(a.Ys), (short)0, a.ys_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t isr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(a.idx + rs), (short)0, n * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t vsr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(a.val + rs), (short)0, n * 4, 0x00020000);
+    // I(j): one buffer load each for the index and the rating of slot l % 32
+    auto load_ids = [&](int j, int &id_raw, float &v_raw) {
+        const int off = 4 * min(CHUNK * j + (lane_here() & 31), n - 1);
+        id_raw = __builtin_amdgcn_raw_buffer_load_b32(isr, off, 0, 0);
+        v_raw = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(vsr, off, 0, 0));
+    };
+    // publish chunk j (parity P): ib[8g + e] = index of rating 4e + g (-1
+    // past the row's end), rb[m] = rating m
+    auto publish = [&](auto parity, int j, int id_raw, float v_raw) {
+        constexpr int P = decltype(parity)::value;
+        int *ib = P ? ib1 : ib0;
+        float *rb = P ? rb1 : rb0;
+        const int m = lane_here() & 31;
+        const bool in = CHUNK * j + m < n;
+        ib[8 * (m & 3) + (m >> 2)] = in ? id_raw : -1;
+        rb[m] = in ? v_raw : 0.f;
+    };
+    // D(j) (parity P): the eight LDS-DMA loads of chunk j into buffer P
+    auto dma = [&](auto parity) {
+        constexpr int P = decltype(parity)::value;
+        const int *ib = P ? ib1 : ib0;
+        uint16_t *dst = P ? buf1 : buf0;
+        wave_sync();
+        if (active) {
+            const int gi = lane_here() / NS;  // = g, recomputed rather than kept
+            const int4 i0 = *reinterpret_cast<const int4 *>(ib + 8 * gi);
+            const int4 i1 = *reinterpret_cast<const int4 *>(ib + 8 * gi + 4);
+            const int ids[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ysr, (lds_void *)(dst + 4 * e * RS), 16,
+                                                         ids[e] * (2 * G::FP) + soff2, 0, 0, 0);
+        }
+    };
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    int id0, id1;
+    float v0, v1;
+    load_ids(0, id0, v0);
+    load_ids(1, id1, v1);
+    publish(P0{}, 0, id0, v0);
+    dma(P0{});
+    load_ids(2, id0, v0);
+    publish(P1{}, 1, id1, v1);
+    dma(P1{});
+    load_ids(3, id1, v1);
+    // Consumes chunk k (parity P = k % 2, static, so the index and rating
+    // registers it publishes are known at compile time) and issues D(k+2)
+    // and I(k+4), unconditionally: a chunk past the row's end has every
+    // slot out of range (index -1), so its loads touch no memory.
+    auto step = [&](auto parity, int k) {
+        constexpr int P = decltype(parity)::value;
+        // D(k) has landed: after it come I(k+2), D(k+1), I(k+3) (12 loads)
+        asm volatile("s_waitcnt vmcnt(12)" ::: "memory");
+        uint16_t *buf = P ? buf1 : buf0;
+        const float *rb = P ? rb1 : rb0;
+        const int nv = min(CHUNK, n - CHUNK * k);
+        if (nv < CHUNK) {
+            // rows past the row's end: zero (16-byte pieces; piece p is row
+            // nv + p / NS, segment p % NS)
+            for (int p = l; p < (CHUNK - nv) * NS; p += WAVE)
+                *reinterpret_cast<u32x4 *>(buf + (nv + p / NS) * RS + 8 * (p % NS)) = u32x4{0u, 0u, 0u, 0u};
+        }
+        if (const int lh = lane_here(); lh < CHUNK) buf[lh * RS + f] = to16<E>(rb[lh]);
+        wave_sync();
+        u32x4 F[T];
+#pragma unroll
+        for (int I = 0; I < T; ++I) {
+            const v4i16 t0 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4i16 *)(buf + roff + 16 * I));
+            const v4i16 t1 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4i16 *)(buf + roff + 4 * RS + 16 * I));
+            const u32x2 w0 = __builtin_bit_cast(u32x2, t0), w1 = __builtin_bit_cast(u32x2, t1);
+            F[I] = u32x4{w0[0], w0[1], w1[0], w1[1]};
+        }
+        // chunk k + 2 into this buffer once its operand reads are done
+        if constexpr (P == 0) publish(P0{}, k + 2, id0, v0);
+        else publish(P1{}, k + 2, id1, v1);
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        dma(parity);
+        if constexpr (P == 0) load_ids(k + 4, id0, v0);
+        else load_ids(k + 4, id1, v1);
+#pragma unroll
+        for (int I = 0; I < T; ++I)
+#pragma unroll
+            for (int J = 0; J <= I; ++J) {
+                if constexpr (E == PREC_BF16)
+                    acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, F[J]), __builtin_bit_cast(bf16x8, F[I]), acc[tile_id(I, J)],
+                        0, 0, 0);
+                else
+                    acc[tile_id(I, J)] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(f16x8, F[J]), __builtin_bit_cast(f16x8, F[I]), acc[tile_id(I, J)],
+                        0, 0, 0);
+            }
+    };
+    int k = 0;
+    for (; k + 1 < nch; k += 2) {
+        step(P0{}, k);
+        step(P1{}, k + 1);
+    }
+    if (k < nch) step(P0{}, k);
+    // every load issued here has landed before the stage is used again
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <int T, int E>
+__device__ __forceinline__ void gram_row16_dma(const UpdateArgs &a, int64_t rs, int n,
+                                               f32x4 (&acc)[Geo<T, E>::NT], char *lds) {
+    using G = Geo<T, E>;
+    constexpr int BUF = CHUNK * G::FP;  // halves per buffer
+    if (n <= 0) return;
+    uint16_t *stage = reinterpret_cast<uint16_t *>(lds + G::MISC_FLOATS * 4);
+    gram_dma_loop<T, E>(a, rs, n, acc, stage, stage + BUF, reinterpret_cast<int *>(lds),
+                        reinterpret_cast<int *>(stage + 2 * BUF), reinterpret_cast<float *>(lds) + 32,
+                        reinterpret_cast<float *>(stage + 2 * BUF) + 32);
+}
+
 template <int T, int E>
 __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
                                          f32x4 (&acc)[Geo<T, E>::NT], char *lds) {
@@ -374,6 +546,12 @@ __device__ __forceinline__ void gram_row(const UpdateArgs &a, int64_t rs, int n,
     char *stage = lds + G::MISC_FLOATS * 4;
 #pragma unroll
     for (int t = 0; t < G::NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (G::DMA_GRAM) {
+        if (a.ys_bytes > 0 && a.gram_dma) {
+            gram_row16_dma<T, E>(a, rs, n, acc, lds);
+            return;
+        }
+    }
     if constexpr (E != PREC_FP32 && T <= 9) {
         // f <= 128: at most 16 segments, one task per lane (every f of
         // T <= 8; f = 128 of T = 9, whose rating column is the 129th)

@@ -37,6 +37,7 @@ struct Side {
     int grid = 0;                  // persistent waves of the default update kernel (variant 0),
                                    // at most the engine's grid_max (autotuned per side)
     int variant = 0;               // update kernel variant (1: one wave per SIMD, where built)
+    int gram_dma = 1;              // 16-bit Gram gathers straight to LDS (where built) or not
     int restage_max = 0;           // refinement: restage rows up to this long (else tile copy)
     float *lambda_rows = nullptr;  // per-row lambda (engine-owned copy), nullable
     // Light rows (fewer than dual_limit ratings, and empty rows) taken by the
@@ -275,6 +276,7 @@ UpdateArgs side_args(als_engine *e, const Side &s, const float *Y, const uint16_
         const int64_t bytes = ny * e->ld * (int64_t)sizeof(uint16_t);
         a.ys_bytes = bytes <= INT32_MAX ? (int)bytes : 0;
     }
+    a.gram_dma = s.gram_dma;
     return a;
 }
 
@@ -807,6 +809,7 @@ int als_get_info(als_engine *e, als_info *o) {
     o->dual_rows[1] = e->side[1].dnunits;
     for (int sd = 0; sd < 2; sd++) {
         o->kernel_variant[sd] = e->side[sd].variant;
+        o->gram_gather[sd] = gram_dma_built(e->T, e->prec) ? e->side[sd].gram_dma : 0;
         o->grid_variant1 = e->grid_v1;
     }
     return ALS_OK;
@@ -1027,6 +1030,20 @@ extern "C" int als_set_dual_limit(als_engine *e, int sd, int limit) {
     return build_side(e, s, s.rows, e->hptr[sd].data(), s.split_len, s.batch_work);
 }
 
+// How one side's 16-bit Gram gathers its rows: 1 straight into LDS (LDS-DMA,
+// two chunks in flight; built for gram_dma_built tile counts), 0 through
+// registers (one chunk in flight). The results are identical.
+extern "C" int als_set_gram_gather(als_engine *e, int sd, int mode) {
+    if (!e) return ALS_E_INVALID;
+    if (sd != 0 && sd != 1) return set_err(e, ALS_E_INVALID, "bad side");
+    if (mode != 0 && mode != 1) return set_err(e, ALS_E_INVALID, "mode must be 0 or 1");
+    if (mode == 1 && !gram_dma_built(e->T, e->prec))
+        return set_err(e, ALS_E_UNSUPPORTED, "no LDS-DMA Gram for this precision and tile count");
+    drop_graph(e);
+    e->side[sd].gram_dma = mode;
+    return ALS_OK;
+}
+
 // Autotuner: times one half-sweep per candidate schedule (split length x
 // batch size, around the cost-model defaults) on the engine's own data and
 // keeps the fastest; then the update kernel's grid (fewer persistent waves
@@ -1055,6 +1072,7 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             res.kernel_variant[sd] = s.variant;
             res.grid[sd] = s.grid;
             res.dual_limit[sd] = s.dual_limit;
+            res.gram_gather[sd] = gram_dma_built(e->T, e->prec) ? s.gram_dma : 0;
             continue;
         }
         const int64_t rows = s.rows;
@@ -1170,21 +1188,32 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             s.restage_max = bestr;
             best = std::min(best, bestt);
         }
-        // then the kernel variant (one wave per SIMD against two) on the chosen schedule
-        if (!rc && e->grid_v1 > 0) {
-            const int v0 = s.variant;
-            int bestv = v0;
+        // then the kernel variant (one wave per SIMD against two) and the Gram's
+        // gather (straight to LDS or through registers), together, on the
+        // chosen schedule
+        const bool dma_built = gram_dma_built(e->T, e->prec);
+        if (!rc && (e->grid_v1 > 0 || dma_built)) {
+            const int v0 = s.variant, d0 = s.gram_dma;
+            int bestv = v0, bestd = d0;
             float bestvt = 1e30f;
             for (int v : {v0, 1 - v0}) {
-                s.variant = v;
-                float t;
-                if ((rc = trial(t))) break;
-                if (t < bestvt) {
-                    bestvt = t;
-                    bestv = v;
+                if (v != v0 && e->grid_v1 == 0) continue;
+                for (int dm : {d0, 1 - d0}) {
+                    if (dm != d0 && !dma_built) continue;
+                    s.variant = v;
+                    s.gram_dma = dm;
+                    float t;
+                    if ((rc = trial(t))) break;
+                    if (t < bestvt) {
+                        bestvt = t;
+                        bestv = v;
+                        bestd = dm;
+                    }
                 }
+                if (rc) break;
             }
             s.variant = bestv;
+            s.gram_dma = bestd;
             best = std::min(best, bestvt);
         }
         if (!rc) rc = restore();
@@ -1193,6 +1222,7 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
         res.kernel_variant[sd] = s.variant;
         res.grid[sd] = s.grid;
         res.dual_limit[sd] = s.dual_limit;
+        res.gram_gather[sd] = dma_built ? s.gram_dma : 0;
         dev_free(e, xsave);
         dev_free(e, xssave);
         dev_free(e, fsave);

@@ -44,7 +44,14 @@ struct UpdateArgs {
     const uint16_t *zero16;     // same, 16-bit
     int ys_bytes;               // bytes of the 16-bit Y shadow when below 2^31 (the pipelined
                                 // Gram then gathers with 32-bit buffer offsets), else 0
+    int gram_dma;               // 1: the 16-bit Gram gathers straight to LDS where built
+                                // (gram_dma_built), 0: through registers
 };
+
+// Tile counts and precisions whose update kernels carry the LDS-DMA Gram:
+// a 16-bit Gram, at most 8 tiles (a chunk's gather tasks fit a wave) and
+// T % 4 != 0 (rows of 16T halves keep a transposing read on distinct banks).
+constexpr bool gram_dma_built(int T, int prec) { return prec != PREC_FP32 && T <= 8 && T % 4 != 0; }
 
 // Several models (different f) on one rating matrix, one launch: batch b of
 // the combined queue is batch b - first[k] of model k (first[k] <= b < first[k+1]).

@@ -146,7 +146,7 @@ class Engine:
         return {"split_len": list(r.split_len), "batch_work": list(r.batch_work),
                 "default_ms": list(r.default_ms), "best_ms": list(r.best_ms), "candidates": list(r.candidates),
                 "restage_max": list(r.restage_max), "kernel_variant": list(r.kernel_variant),
-                "grid": list(r.grid), "dual_limit": list(r.dual_limit)}
+                "grid": list(r.grid), "dual_limit": list(r.dual_limit), "gram_gather": list(r.gram_gather)}
 
     def set_schedule(self, side: int, split_len: int = 0, batch_work: int = 0, restage_max: int = -1) -> None:
         self._check(lib.als_set_schedule(self.h, side, split_len, batch_work, restage_max))
@@ -159,6 +159,9 @@ class Engine:
 
     def set_dual_limit(self, side: int, limit: int) -> None:
         self._check(lib.als_set_dual_limit(self.h, side, limit))
+
+    def set_gram_gather(self, side: int, mode: int) -> None:
+        self._check(lib.als_set_gram_gather(self.h, side, mode))
 
     def device_factors(self, side: int) -> int:
         p = C.c_void_p()

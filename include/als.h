@@ -104,6 +104,8 @@ typedef struct {
     int grid_variant1;   /* persistent waves of the one-wave-per-SIMD variant (0: not built) */
     int dual_limit[2];   /* rows with fewer ratings take the dual system, per side
                             (<= dual_max; als_set_dual_limit, autotuned) */
+    int gram_gather[2];  /* 16-bit Gram gathers per side: 1 straight to LDS, 0 through
+                            registers (als_set_gram_gather, autotuned) */
 } als_info;
 
 int als_config_init(als_config *cfg);
@@ -218,6 +220,7 @@ typedef struct {
     int dual_limit[2];                   /* rating count below which rows take the dual
                                             system (dual_max or one or two tiles fewer;
                                             als_set_dual_limit) */
+    int gram_gather[2];                  /* Gram gathers chosen (als_set_gram_gather) */
 } als_tune_result;
 int als_autotune(als_engine *e, int sides, als_tune_result *out);
 /* Applies one side's schedule directly (e.g. a stored als_autotune result):
@@ -238,6 +241,12 @@ int als_set_grid(als_engine *e, int side, int grid);
  * als_info.dual_max. Both systems have the same solution in exact
  * arithmetic. No reference counterpart. */
 int als_set_dual_limit(als_engine *e, int side, int limit);
+/* How one side's 16-bit Gram gathers the fixed factors' rows: 1 straight
+ * into LDS (LDS-DMA loads, two 32-rating chunks in flight per wave; built for
+ * 1-3 and 5-7 tiles), 0 through registers (one chunk in flight; every tile
+ * count). Results are identical; als_autotune times both. No reference
+ * counterpart. */
+int als_set_gram_gather(als_engine *e, int side, int mode);
 
 /* Model groups (no reference counterpart; the reference trains one model per
  * run): up to 4 engines over the same rating matrix (same m, n, nnz and

@@ -119,3 +119,31 @@ def test_one_wave_kernel_variant_equals_default(f):
     err = np.abs(Ug[users] - Uo).max(axis=1) / np.maximum(1.0, np.abs(Uo).max(axis=1))
     print(f, "max rel err", err.max())
     assert err.max() <= 5e-5
+
+
+@pytest.mark.parametrize("f,prec", [(100, E.Prec.FP16), (100, E.Prec.BF16), (64, E.Prec.FP16), (40, E.Prec.FP16)])
+def test_gram_gather_modes_equal(f, prec):
+    """The 16-bit Gram gathers either straight into LDS (LDS-DMA loads, two
+    chunks in flight) or through registers: bit-identical factors after two
+    warm-started sweeps of a netflix-shaped matrix (scaled) with split rows,
+    light rows and partial last chunks, for 7, 5 and 3 tiles. Tile counts
+    without the LDS-DMA Gram (f = 128) and the FP32 Gram refuse mode 1."""
+    d = ca.scaled(ca.netflix_shape(), 0.03).generate()
+    out = []
+    for mode in (1, 0):
+        cfg = E.make_config(f, 0.05, 2, E.SolveType.CHOL, prec, refine=1, refine_warm=True, split_len=1500)
+        with E.Engine(cfg, d) as e:
+            for side in (0, 1):
+                e.set_gram_gather(side, mode)
+            assert list(e.info().gram_gather) == [mode, mode]
+            e.init_factors()
+            e.train(2, with_rmse=False)
+            info = e.info()
+            assert info.failures == 0 and info.nsplit_rows[1] > 0
+            out.append((e.get_factors(0), e.get_factors(1)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for ff, pp in ((128, E.Prec.FP16), (100, E.Prec.FP32)):
+        with E.Engine(E.make_config(ff, 0.05, 1, E.SolveType.CHOL, pp), d) as e:
+            with pytest.raises(ca.AlsError):
+                e.set_gram_gather(0, 1)
+            e.set_gram_gather(0, 0)
