@@ -135,19 +135,46 @@ def run_gpu_restaged(d, f, prec, sweeps, refine):
         return e.get_factors(0), e.get_factors(1), r, e.info()
 
 
+@pytest.mark.parametrize("f", [16, 64, 128])
 @pytest.mark.parametrize("prec,oprec", [(E.Prec.FP16, O.PREC_FP16), (E.Prec.BF16, O.PREC_BF16)])
-def test_mixed_trajectory_matches_exact_oracle(toy, prec, oprec):
+def test_mixed_trajectory_matches_exact_oracle(toy, f, prec, oprec):
     """16-bit Gram (FP16 or BF16), FP32 Cholesky, two refinement steps whose
-    FP64 residual is formed from the FP32 master rows: the correction solves
-    on the 16-bit factor converge to the unrounded system, so the test RMSE
-    of every sweep is within the mixed-precision bar (2e-4) of the exact FP64
-    oracle, also for BF16, whose rounded system is itself further away (the
-    FP64 oracle fed BF16-rounded factors, printed, moves by ~2e-4 here)."""
-    U, V, r, info = run_gpu_restaged(toy, 16, prec, 5, refine=2)
-    Uo, Vo, ro = run_oracle(toy, 16, O.SOLVE_CHOL, 5)
-    _, _, re = run_oracle(toy, 16, O.SOLVE_CHOL, 5, prec=oprec)
+    FP64 residual is formed from the FP32 master rows, at f = 16, 64 and 128
+    on the 1,000 x 500 toy workload: the correction solves on the 16-bit
+    factor converge to the unrounded system, so the test RMSE of every sweep
+    is within the mixed-precision bar (2e-4) of the exact FP64 oracle, also
+    for BF16, whose rounded system is itself further away (the FP64 oracle
+    fed BF16-rounded factors, printed, moves by up to ~2e-4 here). The
+    factors' largest absolute difference from the oracle's after the five
+    sweeps is printed."""
+    U, V, r, info = run_gpu_restaged(toy, f, prec, 5, refine=2)
+    Uo, Vo, ro = run_oracle(toy, f, O.SOLVE_CHOL, 5)
+    _, _, re = run_oracle(toy, f, O.SOLVE_CHOL, 5, prec=oprec)
     print("gpu", r, "\nexact", ro, "\n16-bit-rounded oracle", re, "\ngpu - exact", np.abs(r - ro).max(),
           "rounded - exact", np.abs(re - ro).max(), "factors", np.abs(U - Uo).max(), np.abs(V - Vo).max())
+    assert info.failures == 0
+    assert np.max(np.abs(r - ro)) <= 2e-4
+
+
+@pytest.mark.parametrize("f", [16, 64, 128])
+@pytest.mark.parametrize("prec,oprec", [(E.Prec.FP16, O.PREC_FP16), (E.Prec.BF16, O.PREC_BF16)])
+def test_warm_started_trajectory_matches_oracle(toy, f, prec, oprec):
+    """The measured solver (16-bit Gram, FP32 Cholesky, one refinement step
+    from the previous iterate, its FP64 residual formed from the Gram tiles,
+    light rows through the dual system) at f = 16, 64 and 128 on the toy
+    workload, where every user and most items take the dual system: every
+    sweep's test RMSE is within the mixed-precision bar (2e-4) of the exact
+    FP64 oracle (measured: FP16 up to 2.8e-5, BF16 up to 1.5e-4; its
+    distance from the oracle of the 16-bit-formed system is printed)."""
+    cfg = E.make_config(f, 0.05, 5, E.SolveType.CHOL, prec, refine=1, refine_warm=True)
+    with E.Engine(cfg, toy) as e:
+        e.init_factors()
+        r = e.train(5)
+        info = e.info()
+    _, _, ro = run_oracle(toy, f, O.SOLVE_CHOL, 5)
+    _, _, re = run_oracle(toy, f, O.SOLVE_CHOL, 5, prec=oprec)
+    print("gpu", r, "\nexact", ro, "\n16-bit-rounded oracle", re, "\ngpu - rounded", np.abs(r - re).max(),
+          "gpu - exact", np.abs(r - ro).max())
     assert info.failures == 0
     assert np.max(np.abs(r - ro)) <= 2e-4
 
