@@ -91,10 +91,13 @@ hipError_t launch_dual(int T, int prec, int solve, const UpdateArgs &a, int grid
 // Reference: calculate_square_error (als_model.cu:62-89) + cublasSasum +
 // host sqrt (:1744-1764). Sixteen lanes share one rating: each loads float4
 // slices of u and v, the FP32 dot is reduced across the 16 lanes, and the
-// squared error is accumulated in FP64. Every block writes one partial; a
-// second kernel sums the partials in a fixed order, so the result is
-// deterministic (no float atomics).
+// squared error is accumulated in FP64. A lane group takes RMSE_ILP of its
+// ratings at once (every index and row load issued before the first dot),
+// so that their memory latencies overlap, and adds their errors in rating
+// order. Every block writes one partial; a second kernel sums the partials
+// in a fixed order, so the result is deterministic (no float atomics).
 constexpr int RMSE_THREADS = 256;
+constexpr int RMSE_ILP = 4;
 
 __global__ __launch_bounds__(RMSE_THREADS) void rmse_kernel(RmseArgs a) {
     __shared__ double red[RMSE_THREADS / WAVE];
@@ -102,23 +105,42 @@ __global__ __launch_bounds__(RMSE_THREADS) void rmse_kernel(RmseArgs a) {
     const int64_t group = ((int64_t)blockIdx.x * RMSE_THREADS + threadIdx.x) >> 4;
     const int64_t ngroups = (int64_t)gridDim.x * (RMSE_THREADS / 16);
     double acc = 0.0;
-    for (int64_t t = group; t < a.n; t += ngroups) {
-        const float *u = a.U + (int64_t)a.row[t] * a.ld;
-        const float *v = a.V + (int64_t)a.col[t] * a.ld;
-        float d = 0.f;
+    for (int64_t t0 = group; t0 < a.n; t0 += RMSE_ILP * ngroups) {
+        const float *u[RMSE_ILP], *v[RMSE_ILP];
+        float r[RMSE_ILP], d[RMSE_ILP];
+        bool in[RMSE_ILP];
+#pragma unroll
+        for (int i = 0; i < RMSE_ILP; ++i) {
+            const int64_t t = t0 + i * ngroups;
+            in[i] = t < a.n;
+            const int64_t tc = in[i] ? t : t0;  // a valid rating, not added
+            u[i] = a.U + (int64_t)a.row[tc] * a.ld;
+            v[i] = a.V + (int64_t)a.col[tc] * a.ld;
+            r[i] = a.val[tc];
+            d[i] = 0.f;
+        }
         for (int j = 4 * sub; j < a.f; j += 64) {
-            const f32x4 x = *reinterpret_cast<const f32x4 *>(u + j);
-            const f32x4 y = *reinterpret_cast<const f32x4 *>(v + j);
-            // columns f.. of the last slice are masked (padding of a caller's rows need not be zero)
-            const float p3 = j + 3 < a.f ? x[3] * y[3] : 0.f, p2 = j + 2 < a.f ? x[2] * y[2] : 0.f;
-            const float p1 = j + 1 < a.f ? x[1] * y[1] : 0.f;
-            d += x[0] * y[0] + p1 + p2 + p3;
+#pragma unroll
+            for (int i = 0; i < RMSE_ILP; ++i) {
+                const f32x4 x = *reinterpret_cast<const f32x4 *>(u[i] + j);
+                const f32x4 y = *reinterpret_cast<const f32x4 *>(v[i] + j);
+                // columns f.. of the last slice are masked (padding of a caller's rows need not be zero)
+                const float p3 = j + 3 < a.f ? x[3] * y[3] : 0.f, p2 = j + 2 < a.f ? x[2] * y[2] : 0.f;
+                const float p1 = j + 1 < a.f ? x[1] * y[1] : 0.f;
+                d[i] += x[0] * y[0] + p1 + p2 + p3;
+            }
         }
 #pragma unroll
-        for (int m = 8; m >= 1; m >>= 1) d += __shfl_xor(d, m, 16);
+        for (int i = 0; i < RMSE_ILP; ++i) {
+#pragma unroll
+            for (int m = 8; m >= 1; m >>= 1) d[i] += __shfl_xor(d[i], m, 16);
+        }
         if (sub == 0) {
-            const double e = (double)a.val[t] - (double)d;
-            acc += e * e;
+#pragma unroll
+            for (int i = 0; i < RMSE_ILP; ++i) {
+                const double e = (double)r[i] - (double)d[i];
+                if (in[i]) acc += e * e;
+            }
         }
     }
     acc = wave_sum_d(acc);
