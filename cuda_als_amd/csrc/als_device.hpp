@@ -391,13 +391,15 @@ __device__ __forceinline__ void gram_row16_piped(const UpdateArgs &a, int64_t rs
 // LDS-DMA loads, issued as I0 I1 D0 I2 D1 I3, then per chunk k consumed:
 // D(k+2) and I(k+4), always (past the row's end they touch no memory), so
 // that when chunk k is consumed the operations after D(k) are I(k+2),
-// D(k+1) and I(k+3): s_waitcnt vmcnt(12) means D(k) has landed.
+// D(k+1) and I(k+3): s_waitcnt vmcnt(12) retires D(k) by count. The
+// compiler adds its own wait before the operand reads, for every LDS-DMA
+// load in flight (it cannot tell the two buffers apart), and the loop
+// relies on that wider wait too: with the count alone, the sums were wrong
+// at T = 3, where a step is shortest.
 // Before a buffer is refilled, lgkmcnt(0) makes sure its operand reads are
 // done. Index and rating loads past the row's end read its last rating.
 // The loop of gram_row16_dma, with the two stage buffers and the two index /
-// rating buffer pairs as distinct restrict pointers: the compiler's wait
-// placement for LDS-DMA then tells an operand read of one buffer from the
-// loads still landing in the other.
+// rating buffer pairs as distinct pointers.
 template <int T, int E>
 __device__ __forceinline__ void gram_dma_loop(const UpdateArgs &a, int64_t rs, int n, f32x4 (&acc)[Geo<T, E>::NT],
                                               uint16_t *__restrict__ buf0, uint16_t *__restrict__ buf1,
