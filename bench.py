@@ -156,6 +156,62 @@ def group_leg(torch, E, ca, steps, warmup, autotune=True):
     return out
 
 
+def adjoint_leg(torch, E, ca, steps, warmup):
+    """The differentiable half-sweep at the ml-latest shape (283,228 users x
+    58,098 items, 24,978,100 train ratings, f = 64, per-user lambda, FP16
+    Gram): the forward users half-sweep keeping every row's Cholesky factor
+    (als_forward_dev_ws) and the hand-written adjoint on it
+    (als_adjoint_dev_ws: x-bar, d/d lambda_u, d/dY), each timed with HIP
+    events on the engine's stream. The adjoint's floor is reading the saved
+    factors once from HBM."""
+    import torch.nn.functional as F
+    d = ca.ml_latest_shape().generate()
+    f = 64
+    s = torch.cuda.Stream()
+    torch.cuda.set_stream(s)
+    with E.Engine(E.make_config(f, 0.05, 1, E.SolveType.CHOL, E.Prec.FP16), d) as e:
+        e.set_stream(s.cuda_stream)
+        e.init_factors()
+        ld = e.info().ld
+        dev = torch.device("cuda")
+        Y = F.pad(torch.tensor(e.get_factors(ca.ALS_SIDE_ITEMS), device=dev), (0, ld - f)).contiguous()
+        X = torch.zeros(d.m, ld, device=dev)
+        lam = torch.full((d.m,), 0.05, device=dev)
+        nbytes = e.factor_save_bytes(ca.ALS_SIDE_USERS)
+        Ls = torch.empty(nbytes // 4, device=dev)
+        g = torch.randn(d.m, ld, device=dev) * 1e-3
+        g[:, f:] = 0
+        xbar, lbar, ybar = torch.zeros_like(X), torch.zeros_like(lam), torch.zeros_like(Y)
+
+        def fwd():
+            e.forward_dev_ws(ca.ALS_SIDE_USERS, Y.data_ptr(), X.data_ptr(), lam.data_ptr(), Ls.data_ptr(), nbytes)
+
+        def adj():
+            e.adjoint_dev_ws(ca.ALS_SIDE_USERS, Y.data_ptr(), X.data_ptr(), Ls.data_ptr(), nbytes, g.data_ptr(),
+                             xbar.data_ptr(), lbar.data_ptr(), ybar.data_ptr())
+
+        def timed(fn):
+            for _ in range(warmup):
+                fn()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(s)
+            for _ in range(steps):
+                fn()
+            ev[1].record(s)
+            torch.cuda.synchronize()
+            return ev[0].elapsed_time(ev[1]) / steps
+
+        fwd_ms = timed(fwd)
+        adj_ms = timed(adj)
+        fails = int(e.info().failures)
+    out = {"workload": f"ml-latest-shape {d.m}x{d.n}, {d.nnz} train ratings, f={f}, per-user lambda, fp16 Gram, "
+                       "fp32 Cholesky; forward = users half-sweep keeping the factors, backward = adjoint",
+           "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "saved_factor_bytes": nbytes,
+           "adjoint_saved_factor_read_tb_s": nbytes / (adj_ms * 1e-3) / 1e12, "failures": fails}
+    log(f"[adjoint] {out}")
+    return out
+
+
 def oracle_fixture(d, f):
     """The FP64 oracle's 10-sweep trajectory of this workload
     (tests/golden/netflix_trajectory.json), if it was made from the same
@@ -262,6 +318,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost-model schedule")
     ap.add_argument("--no-group", action="store_true", help="skip the three-model group leg (ml10M shape)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (ml-latest shape)")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (debug only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -373,6 +430,7 @@ def main() -> int:
         "roofline": roofline,
         "multi_model_group": None if a.no_group else group_leg(torch, E, ca, a.steps, a.warmup,
                                                                 not a.no_autotune),
+        "differentiable_half_sweep": None if a.no_adjoint else adjoint_leg(torch, E, ca, a.steps, a.warmup),
         "cpu_baseline": None if a.no_cpu else cpu_baseline(d, f, lam),
     }
     print(json.dumps(line), flush=True)

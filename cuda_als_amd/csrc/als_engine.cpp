@@ -87,6 +87,12 @@ struct als_engine {
     uint16_t *zero16 = nullptr;
     uint16_t *tmp16 = nullptr;         // shadow of caller-provided Y (forward_dev)
     size_t tmp16_bytes = 0;
+    // the adjoint's dL/dY per side: the other side's rating lists cut into
+    // parts (YbarArgs), built at the side's first adjoint call
+    int4 *yb_parts[2] = {nullptr, nullptr};
+    int *yb_colpart[2] = {nullptr, nullptr};
+    float *yb_partial[2] = {nullptr, nullptr};
+    int yb_nparts[2] = {0, 0};
     hipGraphExec_t graph = nullptr;
     std::vector<int> hptr[2];          // host copies of the CSR / CSC row pointers (schedules)
     std::vector<void *> allocs;
@@ -597,6 +603,15 @@ int als_set_train(als_engine *e, int64_t m, int64_t n, int64_t nnz, const int *c
     if (e->Vs) HIPCHK(e, hipMemsetAsync(e->Vs, 0, (size_t)std::max<int64_t>(1, n * e->ld) * 2, e->stream));
     e->hptr[0].assign(csr_ptr, csr_ptr + m + 1);
     e->hptr[1].assign(csc_ptr, csc_ptr + n + 1);
+    for (int sd = 0; sd < 2; ++sd) {  // the adjoint's parts follow the rating lists
+        dev_free(e, e->yb_parts[sd]);
+        dev_free(e, e->yb_colpart[sd]);
+        dev_free(e, e->yb_partial[sd]);
+        e->yb_parts[sd] = nullptr;
+        e->yb_colpart[sd] = nullptr;
+        e->yb_partial[sd] = nullptr;
+        e->yb_nparts[sd] = 0;
+    }
     if ((rc = build_side(e, su, m, csr_ptr, e->cfg.split_len, e->cfg.batch_work)) ||
         (rc = build_side(e, si, n, csc_ptr, e->cfg.split_len, e->cfg.batch_work)))
         return rc;
@@ -889,7 +904,35 @@ int adjoint_impl(als_engine *e, int sd, const float *d_Y, const float *d_X, cons
     a.f = e->cfg.f;
     if (rows > 0) HIPCHK(e, launch_adjoint_solve(e->T, a, e->stream));
     if (d_ybar && cols > 0) {
+        if (!e->yb_parts[sd]) {
+            // parts of at most YBAR_PART ratings of every column, columns in order
+            const std::vector<int> &hp = e->hptr[1 - sd];
+            std::vector<int4> parts;
+            std::vector<int> colpart(cols + 1);
+            for (int64_t c = 0; c < cols; ++c) {
+                colpart[c] = (int)parts.size();
+                for (int k = hp[c]; k < hp[c + 1]; k += YBAR_PART)
+                    parts.push_back(int4{(int)c, k, std::min(hp[c + 1], k + YBAR_PART), 0});
+            }
+            colpart[cols] = (int)parts.size();
+            int rc;
+            if ((rc = dev_alloc(e, &e->yb_parts[sd], parts.size())) ||
+                (rc = dev_alloc(e, &e->yb_colpart[sd], colpart.size())) ||
+                (rc = dev_alloc(e, &e->yb_partial[sd], std::max<size_t>(1, parts.size()) * e->ld)))
+                return rc;
+            if (!parts.empty())
+                HIPCHK(e, hipMemcpyAsync(e->yb_parts[sd], parts.data(), parts.size() * sizeof(int4),
+                                         hipMemcpyHostToDevice, e->stream));
+            HIPCHK(e, hipMemcpyAsync(e->yb_colpart[sd], colpart.data(), colpart.size() * sizeof(int),
+                                     hipMemcpyHostToDevice, e->stream));
+            HIPCHK(e, hipStreamSynchronize(e->stream));
+            e->yb_nparts[sd] = (int)parts.size();
+        }
         YbarArgs y{};
+        y.parts = e->yb_parts[sd];
+        y.colpart = e->yb_colpart[sd];
+        y.partial = e->yb_partial[sd];
+        y.nparts = e->yb_nparts[sd];
         const Side &other = e->side[1 - sd];  // CSC of this side's matrix = CSR of the other side
         y.cptr = other.ptr;
         y.cidx = other.idx;

@@ -228,43 +228,78 @@ __global__ __launch_bounds__(WAVE) void adjoint_solve_kernel(AdjointArgs a) {
 }
 
 // dL/dy_i = sum_{u in Omega(i)} r_ui xbar_u - xbar_u (x_u . y_i) - x_u (xbar_u . y_i)
-// One wave per column i of the rating matrix, walking its CSC list, so every
-// output row has exactly one writer (no atomics). Lane l owns factors
-// l, l + 64, l + 128 (NJ of them: the padded row is 16 T <= 144 wide).
+// Two passes, no atomics, a fixed summation order. ybar_part_kernel: one
+// wave per part of a column's rating list (at most YBAR_PART ratings, so a
+// popular column is spread over many waves); its four 16-lane groups take
+// every fourth rating, lane c of a group holding factors c, c + 16, ...,
+// c + 16(T-1); the two dots are 16-lane sums; the groups' sums are added at
+// the end and written as the part's row. ybar_sum_kernel: one wave per
+// column adds its parts' rows in part order.
 template <int T>
-__global__ __launch_bounds__(WAVE) void ybar_kernel(YbarArgs a) {
+__global__ __launch_bounds__(WAVE) void ybar_part_kernel(YbarArgs a) {
+    constexpr int FP = 16 * T;
+    const int l = lane(), c = l & 15, grp = l >> 4, f = a.f;
+    for (int p = blockIdx.x; p < a.nparts; p += gridDim.x) {
+        const int4 pt = a.parts[p];
+        const int64_t oy = (int64_t)pt.x * FP;
+        float y[T], ac[T];
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+            const int j = 16 * J + c;
+            y[J] = j < f ? a.Y[oy + j] : 0.f;
+            ac[J] = 0.f;
+        }
+        for (int k = pt.y + grp; k < pt.z; k += 4) {
+            const int64_t ox = (int64_t)a.cidx[k] * FP;
+            const float r = a.cval[k];
+            float x[T], b[T], pxy = 0.f, pby = 0.f;
+#pragma unroll
+            for (int J = 0; J < T; ++J) {
+                const int j = 16 * J + c;
+                x[J] = j < f ? a.X[ox + j] : 0.f;
+                b[J] = j < f ? a.Xbar[ox + j] : 0.f;
+                pxy += x[J] * y[J];
+                pby += b[J] * y[J];
+            }
+#pragma unroll
+            for (int m = 8; m >= 1; m >>= 1) {
+                pxy += __shfl_xor(pxy, m, 16);
+                pby += __shfl_xor(pby, m, 16);
+            }
+#pragma unroll
+            for (int J = 0; J < T; ++J) ac[J] += r * b[J] - b[J] * pxy - x[J] * pby;
+        }
+#pragma unroll
+        for (int J = 0; J < T; ++J) {
+            ac[J] += __shfl_xor(ac[J], 16, WAVE);
+            ac[J] += __shfl_xor(ac[J], 32, WAVE);
+        }
+        if (grp == 0) {
+#pragma unroll
+            for (int J = 0; J < T; ++J) a.partial[(int64_t)p * FP + 16 * J + c] = ac[J];
+        }
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(WAVE) void ybar_sum_kernel(YbarArgs a) {
     constexpr int FP = 16 * T, NJ = (FP + WAVE - 1) / WAVE;
     const int l = lane(), f = a.f;
     for (int col = blockIdx.x; col < a.ncols; col += gridDim.x) {
-        const int64_t oy = (int64_t)col * FP;
-        float y[NJ], ac[NJ];
+        float s[NJ];
 #pragma unroll
-        for (int m = 0; m < NJ; ++m) {
-            const int j = WAVE * m + l;
-            y[m] = j < f ? a.Y[oy + j] : 0.f;
-            ac[m] = 0.f;
-        }
-        for (int k = a.cptr[col]; k < a.cptr[col + 1]; ++k) {
-            const int64_t ox = (int64_t)a.cidx[k] * FP;
-            const float r = a.cval[k];
-            float x[NJ], b[NJ], pxy = 0.f, pby = 0.f;
+        for (int m = 0; m < NJ; ++m) s[m] = 0.f;
+        for (int p = a.colpart[col]; p < a.colpart[col + 1]; ++p) {
 #pragma unroll
             for (int m = 0; m < NJ; ++m) {
                 const int j = WAVE * m + l;
-                x[m] = j < f ? a.X[ox + j] : 0.f;
-                b[m] = j < f ? a.Xbar[ox + j] : 0.f;
-                pxy += x[m] * y[m];
-                pby += b[m] * y[m];
+                if (j < FP) s[m] += a.partial[(int64_t)p * FP + j];
             }
-            const float xy = wave_sum(pxy);
-            const float by = wave_sum(pby);
-#pragma unroll
-            for (int m = 0; m < NJ; ++m) ac[m] += r * b[m] - b[m] * xy - x[m] * by;
         }
 #pragma unroll
         for (int m = 0; m < NJ; ++m) {
             const int j = WAVE * m + l;
-            if (j < FP) a.Ybar[oy + j] = j < f ? ac[m] : 0.f;
+            if (j < FP) a.Ybar[(int64_t)col * FP + j] = j < f ? s[m] : 0.f;
         }
     }
 }
@@ -337,9 +372,13 @@ hipError_t launch_adjoint_solve(int T, const AdjointArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_ybar(int T, const YbarArgs &a, hipStream_t s) {
-    const int grid = std::max(1, std::min(a.ncols, 65536));
+    const int gp = std::max(1, std::min(a.nparts, 65536)), gc = std::max(1, std::min(a.ncols, 65536));
     switch (T) {
-#define ALS_YB(TT) case TT: ybar_kernel<TT><<<grid, WAVE, 0, s>>>(a); break;
+#define ALS_YB(TT)                                                   \
+    case TT:                                                         \
+        ybar_part_kernel<TT><<<gp, WAVE, 0, s>>>(a);                 \
+        ybar_sum_kernel<TT><<<gc, WAVE, 0, s>>>(a);                  \
+        break;
     ALS_YB(1) ALS_YB(2) ALS_YB(3) ALS_YB(4) ALS_YB(5) ALS_YB(6) ALS_YB(7) ALS_YB(8) ALS_YB(9)
 #undef ALS_YB
     default: return hipErrorInvalidValue;

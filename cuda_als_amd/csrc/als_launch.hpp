@@ -91,7 +91,15 @@ struct YbarArgs {
     float *Ybar;                // stride 16*T (overwritten)
     int ncols;
     int f;
+    // the columns' rating lists cut into parts of at most YBAR_PART ratings:
+    // part p = {column, first rating, end}; column c's parts are
+    // colpart[c] .. colpart[c+1]-1, in order; partial: one row per part
+    const int4 *parts;
+    const int *colpart;
+    float *partial;             // nparts x 16*T
+    int nparts;
 };
+constexpr int YBAR_PART = 1024;
 
 struct RmseArgs {
     const int *row, *col;
