@@ -212,6 +212,42 @@ def adjoint_leg(torch, E, ca, steps, warmup):
     return out
 
 
+def batched_cg_leg(torch, E, steps, warmup):
+    """The drop-in for the CG library's updateXWithCGHost (als_cg_update_x,
+    and its FP16-A form): 16,384 synthetic SPD systems at f = 100 on
+    caller buffers, 6 iterations from a zero start (no early exit at this
+    tolerance), timed with HIP events; the bound is reading A once."""
+    batch, f = 16384, 100
+    g = torch.Generator(device="cuda").manual_seed(7)
+    s = torch.cuda.Stream()
+    out = {"workload": f"{batch} SPD systems, f={f}, A = M M^T / f + I (M ~ N(0,1)), cg_iters=6, tol=0",
+           "a_bytes_fp32": batch * f * f * 4}
+    with torch.cuda.stream(s):
+        M = torch.randn(batch, f, f, device="cuda", generator=g)
+        A = torch.bmm(M, M.transpose(1, 2).contiguous()) / f + torch.eye(f, device="cuda")
+        del M
+        b = torch.randn(batch, f, device="cuda", generator=g)
+        for tag, AA in (("fp32_a", A), ("fp16_a", A.half())):
+            x = torch.zeros(batch, f, device="cuda")
+            for _ in range(warmup):
+                x.zero_()
+                E.cg_update_x(AA, x, b, 6, 0.0)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ms = 0.0
+            for _ in range(steps):
+                x.zero_()
+                ev[0].record(s)
+                E.cg_update_x(AA, x, b, 6, 0.0)
+                ev[1].record(s)
+                torch.cuda.synchronize()
+                ms += ev[0].elapsed_time(ev[1])
+            ms /= steps
+            nbytes = AA.numel() * AA.element_size()
+            out[tag] = {"ms": ms, "a_tb_s": nbytes / (ms * 1e-3) / 1e12}
+    log(f"[batched cg] {out}")
+    return out
+
+
 def oracle_fixture(d, f):
     """The FP64 oracle's 10-sweep trajectory of this workload
     (tests/golden/netflix_trajectory.json), if it was made from the same
@@ -319,6 +355,7 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost-model schedule")
     ap.add_argument("--no-group", action="store_true", help="skip the three-model group leg (ml10M shape)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (ml-latest shape)")
+    ap.add_argument("--no-batched-cg", action="store_true", help="skip the batched-CG (caller buffers) leg")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (debug only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -431,6 +468,7 @@ def main() -> int:
         "multi_model_group": None if a.no_group else group_leg(torch, E, ca, a.steps, a.warmup,
                                                                 not a.no_autotune),
         "differentiable_half_sweep": None if a.no_adjoint else adjoint_leg(torch, E, ca, a.steps, a.warmup),
+        "batched_cg": None if a.no_batched_cg else batched_cg_leg(torch, E, a.steps, a.warmup),
         "cpu_baseline": None if a.no_cpu else cpu_baseline(d, f, lam),
     }
     print(json.dumps(line), flush=True)

@@ -1459,6 +1459,7 @@ static int cg_update_x(const float *A, const uint16_t *A16, float *x, const floa
     a.A = A;
     a.A16 = A16;
     a.round_a16 = round_a16 ? 1 : 0;
+    a.vec = f % 4 == 0 && (A ? reinterpret_cast<uintptr_t>(A) % 16 == 0 : reinterpret_cast<uintptr_t>(A16) % 8 == 0);
     a.x = x;
     a.b = b;
     a.batch = batch;

@@ -324,14 +324,44 @@ __global__ __launch_bounds__(WAVE) void cg_batch_kernel(CgBatchArgs a) {
             return a.A[k];
         };
         f32x4 acc[NT];
+        if (a.vec) {
+            // lane (q, c) of tile (I, J): row 16I + c, columns 16J + 4q .. +3,
+            // one load (a diagonal tile's entries A[16I+4q+r][16I+c] read as
+            // their mirror A[16I+c][16I+4q+r], A being symmetric); with
+            // f % 4 == 0 the four columns are all inside f or all past it
 #pragma unroll
-        for (int I = 0; I < T; ++I)
+            for (int I = 0; I < T; ++I)
 #pragma unroll
-            for (int J = 0; J <= I; ++J)
+                for (int J = 0; J <= I; ++J) {
+                    const int i = 16 * I + c, j = 16 * J + 4 * q;
+                    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+                    if (i < f && j < f) {
+                        const int64_t k = base + (int64_t)i * f + j;
+                        if (a.A16) {
+                            typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
+                            f16x4v h;
+                            __builtin_memcpy(&h, a.A16 + k, sizeof(h));
+                            v = __builtin_convertvector(h, f32x4);
+                        } else {
+                            v = *reinterpret_cast<const f32x4 *>(a.A + k);
+                            if (a.round_a16) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    acc[tile_id(I, J)][r] = (I == J) ? at(16 * I + 4 * q + r, 16 * I + c)
-                                                     : at(16 * I + c, 16 * J + 4 * q + r);
+                                for (int r = 0; r < 4; ++r) v[r] = (float)(_Float16)v[r];
+                            }
+                        }
+                    }
+                    acc[tile_id(I, J)] = v;
+                }
+        } else {
+#pragma unroll
+            for (int I = 0; I < T; ++I)
+#pragma unroll
+                for (int J = 0; J <= I; ++J)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        acc[tile_id(I, J)][r] = (I == J) ? at(16 * I + 4 * q + r, 16 * I + c)
+                                                         : at(16 * I + c, 16 * J + 4 * q + r);
+        }
         float bc[T], xc[T];
 #pragma unroll
         for (int J = 0; J < T; ++J) {

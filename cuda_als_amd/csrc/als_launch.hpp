@@ -75,6 +75,8 @@ struct CgBatchArgs {
     const float *A;             // FP32 systems, or nullptr when A16 is set
     const uint16_t *A16;        // FP16 systems
     int round_a16;              // FP32 systems whose entries are rounded to FP16 on load
+    int vec;                    // f % 4 == 0 and A 16-byte (A16 8-byte) aligned: a lane
+                                // loads its four entries of a tile in one load
     float *x;                   // warm start in, solution out
     const float *b;
     int batch;
