@@ -329,29 +329,32 @@ __global__ __launch_bounds__(WAVE) void cg_batch_kernel(CgBatchArgs a) {
             // one load (a diagonal tile's entries A[16I+4q+r][16I+c] read as
             // their mirror A[16I+c][16I+4q+r], A being symmetric); with
             // f % 4 == 0 the four columns are all inside f or all past it
+            typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
+            auto load = [&](auto src, auto cvt) {
 #pragma unroll
-            for (int I = 0; I < T; ++I)
+                for (int I = 0; I < T; ++I)
 #pragma unroll
-                for (int J = 0; J <= I; ++J) {
-                    const int i = 16 * I + c, j = 16 * J + 4 * q;
-                    f32x4 v = {0.f, 0.f, 0.f, 0.f};
-                    if (i < f && j < f) {
-                        const int64_t k = base + (int64_t)i * f + j;
-                        if (a.A16) {
-                            typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
-                            f16x4v h;
-                            __builtin_memcpy(&h, a.A16 + k, sizeof(h));
-                            v = __builtin_convertvector(h, f32x4);
-                        } else {
-                            v = *reinterpret_cast<const f32x4 *>(a.A + k);
-                            if (a.round_a16) {
-#pragma unroll
-                                for (int r = 0; r < 4; ++r) v[r] = (float)(_Float16)v[r];
-                            }
-                        }
+                    for (int J = 0; J <= I; ++J) {
+                        const int i = 16 * I + c, j = 16 * J + 4 * q;
+                        f32x4 v = {0.f, 0.f, 0.f, 0.f};
+                        if (i < f && j < f) v = cvt(src + base + (int64_t)i * f + j);
+                        acc[tile_id(I, J)] = v;
                     }
-                    acc[tile_id(I, J)] = v;
-                }
+            };
+            if (a.A16) {
+                load(a.A16, [](const uint16_t *p) {
+                    f16x4v h;
+                    __builtin_memcpy(&h, p, sizeof(h));
+                    return __builtin_convertvector(h, f32x4);
+                });
+            } else if (a.round_a16) {
+                load(a.A, [](const float *p) {
+                    f32x4 v = *reinterpret_cast<const f32x4 *>(p);
+                    return __builtin_convertvector(__builtin_convertvector(v, f16x4v), f32x4);
+                });
+            } else {
+                load(a.A, [](const float *p) { return *reinterpret_cast<const f32x4 *>(p); });
+            }
         } else {
 #pragma unroll
             for (int I = 0; I < T; ++I)
