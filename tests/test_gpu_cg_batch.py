@@ -89,6 +89,22 @@ def test_cg_batch_fp16_matrix():
     assert rel(xg, run_oracle(Ah, b, x0, 6, 0.0)) <= 1e-4
 
 
+def test_cg_batch_unaligned_matrix_takes_the_entry_loads():
+    """f % 4 == 0 with A 16-byte aligned loads a lane's four entries of a
+    tile at once; an A one float off that alignment (a view into a larger
+    buffer) takes the entry-by-entry loads: the same solutions."""
+    rng = np.random.default_rng(11)
+    A, b, x0 = spd_batch(rng, 48, 64)
+    A = 0.5 * (A + A.transpose(0, 2, 1))   # exactly symmetric: both read orders see the same entries
+    big = torch.zeros(A.size + 1, device="cuda")
+    big[1:] = torch.from_numpy(A.ravel()).cuda()
+    dA = big[1:].view(A.shape)
+    assert dA.data_ptr() % 16 != 0
+    dx = torch.from_numpy(x0).cuda()
+    E.cg_update_x(dA, dx, torch.from_numpy(b).cuda(), 6, 0.0)
+    assert np.array_equal(dx.cpu().numpy(), run_gpu(A, b, x0, 6, 0.0))
+
+
 def test_cg_batch_rejects_bad_arguments():
     A = torch.zeros((2, 144, 144), device="cuda")
     x = torch.zeros((2, 144), device="cuda")
