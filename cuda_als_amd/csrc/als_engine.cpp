@@ -911,21 +911,34 @@ int adjoint_impl(als_engine *e, int sd, const float *d_Y, const float *d_X, cons
             std::vector<int> colpart(cols + 1);
             for (int64_t c = 0; c < cols; ++c) {
                 colpart[c] = (int)parts.size();
-                for (int k = hp[c]; k < hp[c + 1]; k += YBAR_PART)
-                    parts.push_back(int4{(int)c, k, std::min(hp[c + 1], k + YBAR_PART), 0});
+                for (int64_t k = hp[c]; k < hp[c + 1]; k += YBAR_PART)
+                    parts.push_back(int4{(int)c, (int)k, (int)std::min<int64_t>(hp[c + 1], k + YBAR_PART), 0});
             }
             colpart[cols] = (int)parts.size();
-            int rc;
-            if ((rc = dev_alloc(e, &e->yb_parts[sd], parts.size())) ||
-                (rc = dev_alloc(e, &e->yb_colpart[sd], colpart.size())) ||
-                (rc = dev_alloc(e, &e->yb_partial[sd], std::max<size_t>(1, parts.size()) * e->ld)))
-                return rc;
-            if (!parts.empty())
-                HIPCHK(e, hipMemcpyAsync(e->yb_parts[sd], parts.data(), parts.size() * sizeof(int4),
-                                         hipMemcpyHostToDevice, e->stream));
-            HIPCHK(e, hipMemcpyAsync(e->yb_colpart[sd], colpart.data(), colpart.size() * sizeof(int),
-                                     hipMemcpyHostToDevice, e->stream));
-            HIPCHK(e, hipStreamSynchronize(e->stream));
+            // kept only once every buffer is allocated and filled
+            int4 *dparts = nullptr;
+            int *dcolpart = nullptr;
+            float *dpartial = nullptr;
+            int rc = dev_alloc(e, &dparts, parts.size());
+            if (!rc) rc = dev_alloc(e, &dcolpart, colpart.size());
+            if (!rc) rc = dev_alloc(e, &dpartial, std::max<size_t>(1, parts.size()) * e->ld);
+            hipError_t st = hipSuccess;
+            if (!rc && !parts.empty())
+                st = hipMemcpyAsync(dparts, parts.data(), parts.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                    e->stream);
+            if (!rc && st == hipSuccess)
+                st = hipMemcpyAsync(dcolpart, colpart.data(), colpart.size() * sizeof(int), hipMemcpyHostToDevice,
+                                    e->stream);
+            if (!rc && st == hipSuccess) st = hipStreamSynchronize(e->stream);
+            if (rc || st != hipSuccess) {
+                dev_free(e, dparts);
+                dev_free(e, dcolpart);
+                dev_free(e, dpartial);
+                return rc ? rc : set_err(e, ALS_E_HIP, std::string("adjoint work table: ") + hipGetErrorString(st));
+            }
+            e->yb_parts[sd] = dparts;
+            e->yb_colpart[sd] = dcolpart;
+            e->yb_partial[sd] = dpartial;
             e->yb_nparts[sd] = (int)parts.size();
         }
         YbarArgs y{};
