@@ -33,6 +33,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "ALS full sweeps/s @100M nnz f=100, mixed-prec vs FP32 at equal test RMSE"
 PEAK_FP16_TFLOPS = 2500.0     # MI355X dense FP16/BF16 MFMA
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 matrix
+PEAK_HBM_TBS = 8.0            # MI355X HBM3E
 
 
 def log(*a):
@@ -438,6 +439,16 @@ def main() -> int:
                 "kernel_ms": ms, "flops_per_launch": gram_flops,
                 "solve_fp32_flops_per_launch": solve_flops, "roofline_ms_gram_fp16_plus_solve_fp32": roof_ms,
                 "frac_time": roof_ms / ms}
+    # The other half beside it, against the memory roof: its algorithmic gather bytes (a 16-bit
+    # factor row of the fixed side plus an index and a rating per rating) over its time.
+    other, oms = ("users", um) if half == "items" else ("items", im)
+    gather_bytes = float(d.nnz) * (2 * f + 8)
+    gbs = gather_bytes / (oms * 1e-3) / 1e12
+    roofline_other = {"half": other, "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_TBS, "unit": "TB/s",
+                      "frac": gbs / PEAK_HBM_TBS, "traffic": parse_traffic(f"update_{other}"),
+                      "kernel_ms": oms, "bytes_per_launch": gather_bytes,
+                      "note": "the fixed side's 16-bit rows largely come from the Infinity Cache and L2, so "
+                              "traffic (L2 misses, from the committed profile) can sit below the algorithmic bytes"}
 
     line = {
         "metric": METRIC, "value": a.steps / M["dt"], "unit": "sweeps/s", "n_gpus": 1, "steps": a.steps,
@@ -465,6 +476,7 @@ def main() -> int:
         "test_rmse_delta_vs_oracle": ({"mixed": M["vs_oracle"], "fp32": fp32 and fp32["vs_oracle"],
                                        "fp32_cg6": cg6 and cg6["vs_oracle"]} if fix else None),
         "roofline": roofline,
+        "roofline_other_half": roofline_other,
         "multi_model_group": None if a.no_group else group_leg(torch, E, ca, a.steps, a.warmup,
                                                                 not a.no_autotune),
         "differentiable_half_sweep": None if a.no_adjoint else adjoint_leg(torch, E, ca, a.steps, a.warmup),
