@@ -121,6 +121,32 @@ def test_one_wave_kernel_variant_equals_default(f):
     assert err.max() <= 5e-5
 
 
+@pytest.mark.parametrize("f,prec", [(100, E.Prec.FP16), (97, E.Prec.BF16), (120, E.Prec.FP16), (128, E.Prec.FP16)])
+def test_one_wave_register_gathers_equal_default(f, prec):
+    """The one-wave variant with the register-gather Gram (the Gram the
+    autotuner picks for the items half): bit-identical factors to the
+    default kernel with register gathers, after two warm-started sweeps of a
+    netflix-shaped matrix (scaled) whose rows end on every chunk position
+    (rows shorter than a chunk, one and two chunks, partial last chunks),
+    for 7, 8 and 9 tiles and both 16-bit formats."""
+    d = ca.scaled(ca.netflix_shape(), 0.03).generate()
+    out = []
+    for variant in (0, 1):
+        cfg = E.make_config(f, 0.05, 1, E.SolveType.CHOL, prec, refine=1, refine_warm=True, split_len=2048)
+        with E.Engine(cfg, d) as e:
+            for side in (0, 1):
+                e.set_kernel_variant(side, variant)
+                e.set_gram_gather(side, 0)
+            e.init_factors()
+            e.train(2, with_rmse=False)
+            out.append((e.get_factors(0), e.get_factors(1), e.info()))
+    (U0, V0, i0), (U1, V1, i1) = out
+    assert list(i1.kernel_variant) == [1, 1] and list(i1.gram_gather) == [0, 0]
+    assert i0.failures == 0 and i1.failures == 0
+    assert np.all(np.isfinite(U1)) and np.abs(U1).max() > 0
+    assert np.array_equal(U0, U1) and np.array_equal(V0, V1)
+
+
 @pytest.mark.parametrize("f,prec", [(100, E.Prec.FP16), (100, E.Prec.BF16), (64, E.Prec.FP16), (40, E.Prec.FP16)])
 def test_gram_gather_modes_equal(f, prec):
     """The 16-bit Gram gathers either straight into LDS (LDS-DMA loads, two
