@@ -1249,38 +1249,33 @@ __device__ __forceinline__ float colform_dot(const float *a, const float *b, int
 //   diagonal (I,I): y[16I+4q+r] += A v[16I+c]        row-form partial
 //   off (I,J):      y[16I+c]    += A v[16J+4q+r]     column-form partial
 //                   y[16J+4q+r] += A v[16I+c]        row-form partial
+// Block column J outermost: block J's row-form partial (its diagonal tile,
+// then the tiles below it) and the column-form partials of the blocks below
+// are complete at the end of step J, so y[J] is formed there and only one
+// row-form partial and one row form of v are live at a time. Each partial
+// sums its terms in ascending block order.
 template <int T, int NT>
 __device__ __forceinline__ void matvec_f32(const f32x4 (&A)[NT], int f, const float (&v)[T], float (&y)[T]) {
     const int c = lane() & 15;
-    f32x4 vr[T];
-#pragma unroll
-    for (int I = 0; I < T; ++I) vr[I] = col_to_row(v[I]);
-    f32x4 pr[T];
     float pc[T];
 #pragma unroll
-    for (int I = 0; I < T; ++I) {
-        pr[I] = f32x4{0.f, 0.f, 0.f, 0.f};
-        pc[I] = 0.f;
-    }
+    for (int I = 0; I < T; ++I) pc[I] = 0.f;
 #pragma unroll
-    for (int I = 0; I < T; ++I)
+    for (int J = 0; J < T; ++J) {
+        const f32x4 vr = col_to_row(v[J]);
+        f32x4 pr = f32x4{0.f, 0.f, 0.f, 0.f};
+        pr += A[tile_id(J, J)] * v[J];
 #pragma unroll
-        for (int J = 0; J <= I; ++J) {
+        for (int I = J + 1; I < T; ++I) {
             const f32x4 t = A[tile_id(I, J)];
-            if (J == I) {
-                pr[I] += t * v[I];
-            } else {
-                pc[I] += t[0] * vr[J][0] + t[1] * vr[J][1] + t[2] * vr[J][2] + t[3] * vr[J][3];
-                pr[J] += t * v[I];
-            }
+            pc[I] += t[0] * vr[0] + t[1] * vr[1] + t[2] * vr[2] + t[3] * vr[3];
+            pr += t * v[I];
         }
-#pragma unroll
-    for (int I = 0; I < T; ++I) {
         f32x4 s;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) s[r] = row16_sum(pr[I][r]);
-        const float yc = row_to_col(s) + q4_sum(pc[I]);
-        y[I] = (16 * I + c < f) ? yc : 0.f;
+        for (int r = 0; r < 4; ++r) s[r] = row16_sum(pr[r]);
+        const float yc = row_to_col(s) + q4_sum(pc[J]);
+        y[J] = (16 * J + c < f) ? yc : 0.f;
     }
 }
 

@@ -309,8 +309,13 @@ __global__ __launch_bounds__(WAVE) void ybar_sum_kernel(YbarArgs a) {
 // tiles row by lane A[16I+c][16J+4q+r]), zero outside the f x f block,
 // then the same register CG as the fused update (cumf recurrence: warm
 // start, cg_iters cap, exit once r.r < tol after an update, pAp == 0 guard).
+// Up to 7 tiles two waves per SIMD (256 VGPRs; at 7 tiles a few loop-
+// invariant values spill, none inside the iterations), so that one
+// system's loads and reduction chains run beside another's; 8 and 9 tiles
+// keep the whole register file.
 template <int T>
-__global__ __launch_bounds__(WAVE) void cg_batch_kernel(CgBatchArgs a) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(T <= 7 ? 2 : 1))) void cg_batch_kernel(
+    CgBatchArgs a) {
     constexpr int NT = T * (T + 1) / 2;
     const int l = lane(), q = l >> 4, c = l & 15;
     const int f = a.f;
