@@ -334,31 +334,47 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(T <= 7 ? 2
             // one load (a diagonal tile's entries A[16I+4q+r][16I+c] read as
             // their mirror A[16I+c][16I+4q+r], A being symmetric); with
             // f % 4 == 0 the four columns are all inside f or all past it
+            // Every tile's load is issued before any value is converted:
+            // unconditional buffer loads through a descriptor over this
+            // system's A, a tile past f reading zeros by the range check (a
+            // bounds test around each load would hold its conversion and
+            // wait for the load there, one memory round trip per tile). An
+            // FP16 tile lands in the low half of its accumulator's bits and
+            // is widened in place afterwards.
             typedef _Float16 f16x4v __attribute__((ext_vector_type(4)));
-            auto load = [&](auto src, auto cvt) {
+            const int esz = a.A16 ? 2 : 4;
+            const void *sysA = a.A16 ? (const void *)(a.A16 + base) : (const void *)(a.A + base);
+            const __amdgpu_buffer_rsrc_t rsA =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(sysA), (short)0, f * f * esz, 0x00020000);
+            auto off = [&](int I, int J) {
+                const int i = 16 * I + c, j = 16 * J + 4 * q;
+                return (i < f && j < f) ? (i * f + j) * esz : 0x40000000;  // f * f * esz < 2^17
+            };
+            if (a.A16) {
 #pragma unroll
                 for (int I = 0; I < T; ++I)
 #pragma unroll
                     for (int J = 0; J <= I; ++J) {
-                        const int i = 16 * I + c, j = 16 * J + 4 * q;
-                        f32x4 v = {0.f, 0.f, 0.f, 0.f};
-                        if (i < f && j < f) v = cvt(src + base + (int64_t)i * f + j);
-                        acc[tile_id(I, J)] = v;
+                        const u32x2 w = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rsA, off(I, J), 0, 0));
+                        acc[tile_id(I, J)] = __builtin_bit_cast(f32x4, u32x4{w[0], w[1], 0u, 0u});
                     }
-            };
-            if (a.A16) {
-                load(a.A16, [](const uint16_t *p) {
-                    f16x4v h;
-                    __builtin_memcpy(&h, p, sizeof(h));
-                    return __builtin_convertvector(h, f32x4);
-                });
-            } else if (a.round_a16) {
-                load(a.A, [](const float *p) {
-                    f32x4 v = *reinterpret_cast<const f32x4 *>(p);
-                    return __builtin_convertvector(__builtin_convertvector(v, f16x4v), f32x4);
-                });
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const u32x4 u = __builtin_bit_cast(u32x4, acc[t]);
+                    acc[t] = __builtin_convertvector(__builtin_bit_cast(f16x4v, u32x2{u[0], u[1]}), f32x4);
+                }
             } else {
-                load(a.A, [](const float *p) { return *reinterpret_cast<const f32x4 *>(p); });
+#pragma unroll
+                for (int I = 0; I < T; ++I)
+#pragma unroll
+                    for (int J = 0; J <= I; ++J)
+                        acc[tile_id(I, J)] =
+                            __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rsA, off(I, J), 0, 0));
+                if (a.round_a16) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[t] = __builtin_convertvector(__builtin_convertvector(acc[t], f16x4v), f32x4);
+                }
             }
         } else {
 #pragma unroll
