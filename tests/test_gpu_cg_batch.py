@@ -89,6 +89,27 @@ def test_cg_batch_fp16_matrix():
     assert rel(xg, run_oracle(Ah, b, x0, 6, 0.0)) <= 1e-4
 
 
+@pytest.mark.parametrize("f", [40, 100, 143])
+def test_cg_batch_fp32_matrix_rounded_to_fp16(f):
+    """als_cg_update_x_tt_fp16 (FP32 A, each entry rounded to FP16 in the
+    kernel): the same solutions as the FP16-A entry on A.half() (both round
+    to nearest even), bit for bit, and the oracle's recurrence on the
+    rounded A within the FP32 bar; f = 143 takes the entry-by-entry loads."""
+    import ctypes as C
+    from cuda_als_amd._capi import lib
+    rng = np.random.default_rng(f + 3)
+    A, b, x0 = spd_batch(rng, 64, f)
+    dA, db = torch.from_numpy(A).cuda(), torch.from_numpy(b).cuda()
+    dx = torch.from_numpy(x0).cuda()
+    rc = lib.als_cg_update_x_tt_fp16(C.c_void_p(dA.data_ptr()), C.c_void_p(dx.data_ptr()), C.c_void_p(db.data_ptr()),
+                                     64, f, 6, 0.0, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    xt = dx.cpu().numpy()
+    assert np.array_equal(xt, run_gpu(A, b, x0, 6, 0.0, half=True))
+    Ah = A.astype(np.float16).astype(np.float32)
+    assert rel(xt, run_oracle(Ah, b, x0, 6, 0.0)) <= 1e-4
+
+
 def test_cg_batch_unaligned_matrix_takes_the_entry_loads():
     """f % 4 == 0 with A 16-byte aligned loads a lane's four entries of a
     tile at once; an A one float off that alignment (a view into a larger
