@@ -45,7 +45,7 @@ def rel(a, ref):
     return (np.abs(a - ref).max(axis=1) / np.maximum(1.0, np.abs(ref).max(axis=1))).max()
 
 
-@pytest.mark.parametrize("f", [1, 7, 16, 33, 100, 143])
+@pytest.mark.parametrize("f", [1, 7, 16, 33, 100, 112, 128, 143])
 def test_cg_batch_matches_oracle_recurrence(f):
     rng = np.random.default_rng(f)
     A, b, x0 = spd_batch(rng, 96, f)
@@ -89,12 +89,13 @@ def test_cg_batch_fp16_matrix():
     assert rel(xg, run_oracle(Ah, b, x0, 6, 0.0)) <= 1e-4
 
 
-@pytest.mark.parametrize("f", [40, 100, 143])
+@pytest.mark.parametrize("f", [40, 100, 128, 143])
 def test_cg_batch_fp32_matrix_rounded_to_fp16(f):
     """als_cg_update_x_tt_fp16 (FP32 A, each entry rounded to FP16 in the
     kernel): the same solutions as the FP16-A entry on A.half() (both round
     to nearest even), bit for bit, and the oracle's recurrence on the
-    rounded A within the FP32 bar; f = 143 takes the entry-by-entry loads."""
+    rounded A within the FP32 bar; f = 128 has nine tiles (one wave per
+    SIMD), f = 143 takes the entry-by-entry loads."""
     import ctypes as C
     from cuda_als_amd._capi import lib
     rng = np.random.default_rng(f + 3)
