@@ -1609,9 +1609,20 @@ __device__ __forceinline__ void run_batch(const UpdateArgs &a, int b, char *lds)
                     for (int t = 0; t < G::NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
                     for (int p = 0; p < sp.y; ++p) {
                         const float *src = a.partial + (int64_t)(sp.x + p) * G::NT * 256;
+                        // T tiles' loads in flight before their adds (one
+                        // register reused for every tile would wait for each
+                        // load in turn)
 #pragma unroll
-                        for (int t = 0; t < G::NT; ++t)
-                            acc[t] += *reinterpret_cast<const f32x4 *>(src + t * 256 + 4 * l);
+                        for (int t0 = 0; t0 < G::NT; t0 += T) {
+                            f32x4 part_t[T];
+#pragma unroll
+                            for (int k = 0; k < T; ++k)
+                                if (t0 + k < G::NT)
+                                    part_t[k] = *reinterpret_cast<const f32x4 *>(src + (t0 + k) * 256 + 4 * l);
+#pragma unroll
+                            for (int k = 0; k < T; ++k)
+                                if (t0 + k < G::NT) acc[t0 + k] += part_t[k];
+                        }
                     }
                     if (l == 0) atomicExch(a.counters + w.w, 0);
                 }

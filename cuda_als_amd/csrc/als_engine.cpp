@@ -1101,8 +1101,9 @@ extern "C" int als_set_gram_gather(als_engine *e, int sd, int mode) {
 }
 
 // Autotuner: times one half-sweep per candidate schedule (split length x
-// batch size, around the cost-model defaults) on the engine's own data and
-// keeps the fastest; then the update kernel's grid (fewer persistent waves
+// batch size, around the cost-model defaults, under each kernel variant the
+// engine has) on the engine's own data and keeps the fastest; then the
+// update kernel's grid (fewer persistent waves
 // than the occupancy limit can gather faster: the items half of the
 // netflix-shaped workload runs 4% faster at six waves per CU than at
 // eight), the shape of the light rows' systems (the rating count below which
@@ -1178,18 +1179,30 @@ extern "C" int als_autotune(als_engine *e, int sides, als_tune_result *out) {
             }
             return ALS_OK;
         };
+        // split length x batch size, under each kernel variant the engine
+        // has (the best split length depends on it: a variant's waves take
+        // a split row's parts and finishes at their own rate)
         float best = 1e30f;
         size_t besti = 0;
-        for (size_t i = 0; i < cand.size() && !rc; i++) {
-            if ((rc = build_side(e, s, rows, hp, cand[i].first, cand[i].second))) break;
-            float t;
-            if ((rc = trial(t))) break;
-            if (i == 0) res.default_ms[sd] = t;
-            if (t < best) {
-                best = t;
-                besti = i;
+        const int sv0 = s.variant;
+        int bestsv = sv0;
+        for (int v : {sv0, 1 - sv0}) {
+            if (v != sv0 && e->grid_v1 == 0) continue;
+            s.variant = v;
+            for (size_t i = 0; i < cand.size() && !rc; i++) {
+                if ((rc = build_side(e, s, rows, hp, cand[i].first, cand[i].second))) break;
+                float t;
+                if ((rc = trial(t))) break;
+                if (i == 0 && v == sv0) res.default_ms[sd] = t;
+                if (t < best) {
+                    best = t;
+                    besti = i;
+                    bestsv = v;
+                }
             }
+            if (rc) break;
         }
+        s.variant = bestsv;
         if (!rc) rc = build_side(e, s, rows, hp, cand[besti].first, cand[besti].second);
         // then the grid: the occupancy limit and up to half as many waves per
         // CU, the schedule rebuilt for each
