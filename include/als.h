@@ -198,7 +198,8 @@ int als_schedule_preview(int64_t rows, const int *ptr, int split_len, int batch_
 /* Autotuner (no reference counterpart: the reference launches one block per
  * row with a fixed shape). For each side selected in `sides` (bit 0 users,
  * bit 1 items) times one half-sweep per candidate schedule, split length x
- * batch size around the cost-model defaults, then the update kernel's grid,
+ * batch size around the cost-model defaults (under each kernel variant
+ * built), then the update kernel's grid,
  * the dual-system limit (which rows solve n x n instead of f x f systems),
  * (with refinement) the residual's restage threshold and the kernel variant,
  * on the engine's data, and keeps the fastest. Factors and the failure count
