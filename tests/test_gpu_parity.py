@@ -518,6 +518,45 @@ def test_autotune_keeps_state_and_results():
         assert np.max(np.abs(V0 - V1)) <= 2e-5 * max(1.0, np.abs(V0).max())
 
 
+def test_autotune_warm_tries_split_lengths_under_each_variant():
+    """With the warm-started solve at 7 tiles (f = 100) both update kernel
+    variants are built, and the autotuner times the split length x batch
+    size candidates under each: it leaves the factors as it found them,
+    picks a built variant, its best time is no worse than the default
+    schedule's, and a sweep on the tuned schedule matches one on the
+    default schedule (a different split length only reorders FP32 sums)."""
+    d = ca.scaled(ca.netflix_shape(), 0.05).generate()
+    cfg = E.make_config(100, 0.05, 1, E.SolveType.CHOL, E.Prec.FP16, refine=1, refine_warm=True)
+    with E.Engine(cfg, d) as e0, E.Engine(cfg, d) as e1:
+        assert e1.info().grid_variant1 > 0
+        e0.init_factors()
+        e1.init_factors()
+        U_before, V_before = e1.get_factors(0), e1.get_factors(1)
+        r = e1.autotune()
+        print(r)
+        assert np.array_equal(e1.get_factors(0), U_before) and np.array_equal(e1.get_factors(1), V_before)
+        assert e1.info().failures == 0
+        for sd in (0, 1):
+            assert r["kernel_variant"][sd] in (0, 1)
+            assert r["best_ms"][sd] <= r["default_ms"][sd]
+        assert list(e1.info().kernel_variant) == r["kernel_variant"]
+        assert list(e1.info().split_len) == r["split_len"]
+        # The tuned dual limit moves rows between the dual system (a plain
+        # FP32 solve) and the primal warm-started step, which from the
+        # random initial factors differ by up to ~1e-4 on the lightest,
+        # ill-conditioned items. For the comparison both engines keep the
+        # default limit.
+        for sd in (0, 1):
+            e1.set_dual_limit(sd, e0.info().dual_limit[sd])
+        e0.sweep()
+        e1.sweep()
+        U0, V0, U1, V1 = e0.get_factors(0), e0.get_factors(1), e1.get_factors(0), e1.get_factors(1)
+        assert e1.info().failures == 0
+        print("tuned vs default", np.abs(U0 - U1).max(), np.abs(V0 - V1).max())
+        assert np.max(np.abs(U0 - U1)) <= 2e-5 * max(1.0, np.abs(U0).max())
+        assert np.max(np.abs(V0 - V1)) <= 2e-5 * max(1.0, np.abs(V0).max())
+
+
 @pytest.mark.parametrize("prec,refine,warm", [(E.Prec.BF16, 0, False), (E.Prec.FP16, 0, False),
                                               (E.Prec.FP16, 1, True)])
 def test_group_single_launch_equals_separate_runs(prec, refine, warm):
